@@ -1,0 +1,94 @@
+/* libqpx -- batched dense convex QP solve (primal-dual interior point) and its implicit-
+ * differentiation backward on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI: plain pointers and sizes only.  All array arguments are DEVICE pointers to row-major,
+ * contiguous arrays with a leading batch axis, in the element type named by `dtype`
+ * (QPX_F32 = float, QPX_F64 = double); flags are int32.  `stream` is a hipStream_t (NULL = the
+ * default stream).  Every call is stream-ordered; the caller owns every buffer, including the
+ * workspace, and the library keeps no pointer past the call.
+ *
+ * Problem (per batch entry):   minimise 1/2 x'Qx + q'x   s.t.  A x = b,  G x <= h
+ *   Q (n,n)  q (n)  A (m_eq,n)  b (m_eq)  G (m_ineq,n)  h (m_ineq)
+ *
+ * Numerical failure is not an error: it is reported per problem through converged = 0, as the
+ * reference does.  Errors (bad arguments, HIP failures) return a non-zero status and leave a
+ * message for qpx_last_error().
+ *
+ * Modes: QPX_PLAIN runs one kernel per IPM stage per iteration and checks convergence on the host
+ * (one device->host read per iteration); QPX_FUSED runs the whole loop in one kernel launch per
+ * phase with no host synchronisation (capturable in a hipGraph).  Both give identical results.
+ */
+#ifndef QPX_H_
+#define QPX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPX_API __attribute__((visibility("default")))
+
+typedef enum qpx_status {
+  QPX_OK = 0,
+  QPX_ERR_ARG = 1,          /* invalid argument (sizes, null pointers, dtype, mode) */
+  QPX_ERR_HIP = 2,          /* a HIP runtime call failed */
+  QPX_ERR_WORKSPACE = 3,    /* workspace smaller than qpx_workspace_size() */
+  QPX_ERR_UNSUPPORTED = 4   /* a size outside what this build supports */
+} qpx_status;
+
+enum { QPX_F32 = 0, QPX_F64 = 1 };
+enum { QPX_PLAIN = 0, QPX_FUSED = 1 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+QPX_API int32_t qpx_version(void);
+
+/* Thread-local text of the last error returned on this thread ("" if none). */
+QPX_API const char* qpx_last_error(void);
+
+/* Bytes of device workspace a call with these sizes needs (any entry point below). */
+QPX_API size_t qpx_workspace_size(int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq);
+
+/* qpax.solve_qp -- replaces qpax/pdip.py:181-251 (solve_qp), the driver of initialize
+ * (pdip.py:15-35), the predictor-corrector loop (pdip.py:87-119, :228-238), inf-bound removal
+ * (pdip.py:166-178, :243-246) and Q symmetrisation (pdip.py:223).  Requires m_ineq >= 1 (the
+ * reference's static branches for m_ineq == 0 are qpx_solve_eq_only / qpx_solve_unconstrained).
+ * Outputs: x (B,n), s (B,m_ineq), z (B,m_ineq), y (B,m_eq), converged (B), iters (B). */
+QPX_API qpx_status qpx_solve_qp(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
+                                int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
+                                const void* A, const void* b, const void* G, const void* h,
+                                double solver_tol, void* x, void* s, void* z, void* y,
+                                int32_t* converged, int32_t* iters, void* workspace,
+                                size_t workspace_bytes, void* stream);
+
+/* Forward of qpax.solve_qp_primal under differentiation -- replaces
+ * solve_qp_primal_forward (qpax/diff_qp.py:61-71): solve_qp, then relax_qp
+ * (qpax/pdip_relaxed.py:70-103, up to 30 damped Newton steps on s*z = target_kappa) from the
+ * solution.  Writes the solve outputs as qpx_solve_qp plus the relaxed point xr, sr, zr, yr and
+ * the relaxation's converged/iters flags. */
+QPX_API qpx_status qpx_solve_qp_relaxed(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
+                                        int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
+                                        const void* A, const void* b, const void* G, const void* h,
+                                        double solver_tol, double target_kappa, void* x, void* s,
+                                        void* z, void* y, int32_t* converged, int32_t* iters,
+                                        void* xr, void* sr, void* zr, void* yr,
+                                        int32_t* relax_converged, int32_t* relax_iters,
+                                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* Backward of qpax.solve_qp_primal -- replaces solve_qp_primal_backward / diff_qp /
+ * optnet_derivatives (qpax/diff_qp.py:8-46, :74-79): factor the KKT system at the relaxed point
+ * (sr, zr), solve it with right-hand side (-dl_dx, 0, 0, 0) and form the six cotangents
+ * dQ (B,n,n), dq (B,n), dA (B,m_eq,n), db (B,m_eq), dG (B,m_ineq,n), dh (B,m_ineq). */
+QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
+                               int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
+                               const void* A, const void* b, const void* G, const void* h,
+                               const void* xr, const void* sr, const void* zr, const void* yr,
+                               const void* dl_dx, void* dQ, void* dq, void* dA, void* db,
+                               void* dG, void* dh, void* workspace, size_t workspace_bytes,
+                               void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QPX_H_ */

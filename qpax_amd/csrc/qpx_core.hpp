@@ -1,0 +1,433 @@
+// Device building blocks of the batched PDIP solver: one thread block owns one QP.
+//
+// Every function below is a block-cooperative stage of qpax's algorithm, written for a block of
+// NT threads (NT a multiple of 64).  All IPM arithmetic is float64 whatever the storage type of
+// the caller's arrays: the reduced KKT matrix H = Q + G^T diag(z/s) G reaches condition numbers
+// far beyond 1/eps(float32) as the iterates approach the boundary, and on gfx950 (packed FP32
+// disabled) a float64 FMA issues at the rate of a float32 one.
+//
+// Linear algebra replaces the reference's QR (qpax/pdip.py:10-12,38-46) by Cholesky factors of
+// the same symmetric positive definite matrices; the reduction itself (cvxopt, Algorithm 1,
+// qpax/pdip.py:49-63) is unchanged.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qpx {
+
+constexpr int kMaxIters = 30;          // qpax/pdip.py:232, pdip_relaxed.py:76
+constexpr double kPivotRel = 1e-14;    // pivot floor relative to the matrix's own diagonal entry
+constexpr double kHugePivot = 1e64;    // a pivot below the floor is replaced by this (drops the direction)
+
+// jnp.min / jnp.max propagate NaN; so do these.
+__device__ __forceinline__ double nmin(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+struct OpMin { __device__ double operator()(double a, double b) const { return nmin(a, b); } };
+struct OpMax { __device__ double operator()(double a, double b) const { return nmax(a, b); } };
+struct OpSum { __device__ double operator()(double a, double b) const { return a + b; } };
+
+// Reduction over the block; every thread receives the result.  `red` is LDS scratch of NT/64
+// doubles.  Contains barriers: call from uniform control flow only.
+template <int NT, class Op>
+__device__ __forceinline__ double block_reduce(double v, double* red, Op op) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+  if constexpr (NT > 64) {
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    v = red[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) v = op(v, red[w]);
+  }
+  return v;
+}
+
+// --------------------------------------------------------------------------------------------
+// Per-problem view of the float64 workspace.
+// --------------------------------------------------------------------------------------------
+enum Ctl : int { kConv = 0, kIter = 1, kActive = 2, kRelaxConv = 3, kRelaxIter = 4, kCtlWords = 8 };
+
+struct View {
+  int n, me, mi;
+  // sanitised problem: Q symmetrised, rows of G and entries of h zeroed where h is +-inf
+  double *Q, *G, *A, *q, *b, *h, *hinf;
+  // iterate
+  double *x, *s, *z, *y;
+  // factorisation: H (n x n, lower = L), W = L^{-1} A^T (n x me, row-major), F (me x me, lower)
+  double *H, *W, *F, *hdiag, *hrcp, *fdiag, *frcp;
+  // work vectors
+  double *D, *dx, *ds, *dz, *dy, *dsa, *dza, *r1, *r2, *r3, *r4, *u, *t;
+  double *scal;  // [0]=sigma*mu, [1]=alpha, [2]=kappa
+  int* ctl;
+};
+
+__host__ __device__ inline int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// Number of float64 words of workspace one problem of size (n, me, mi) occupies.
+__host__ __device__ inline int64_t view_words(int n, int me, int mi) {
+  int64_t w = 0;
+  auto add = [&](int64_t k) { w += round_up(k > 0 ? k : 1, 2); };
+  add((int64_t)n * n); add((int64_t)mi * n); add((int64_t)me * n); add(n); add(me); add(mi); add(mi);
+  add(n); add(mi); add(mi); add(me);
+  add((int64_t)n * n); add((int64_t)n * me); add((int64_t)me * me); add(n); add(n); add(me); add(me);
+  add(mi); add(n); add(mi); add(mi); add(me); add(mi); add(mi); add(n); add(mi); add(mi); add(me); add(n); add(me);
+  add(8);
+  add(kCtlWords / 2);
+  return round_up(w, 64);
+}
+
+__host__ __device__ inline View carve(double* base, int n, int me, int mi) {
+  View v;
+  v.n = n; v.me = me; v.mi = mi;
+  double* p = base;
+  auto take = [&](int64_t k) { double* r = p; p += round_up(k > 0 ? k : 1, 2); return r; };
+  v.Q = take((int64_t)n * n); v.G = take((int64_t)mi * n); v.A = take((int64_t)me * n);
+  v.q = take(n); v.b = take(me); v.h = take(mi); v.hinf = take(mi);
+  v.x = take(n); v.s = take(mi); v.z = take(mi); v.y = take(me);
+  v.H = take((int64_t)n * n); v.W = take((int64_t)n * me); v.F = take((int64_t)me * me);
+  v.hdiag = take(n); v.hrcp = take(n); v.fdiag = take(me); v.frcp = take(me);
+  v.D = take(mi); v.dx = take(n); v.ds = take(mi); v.dz = take(mi); v.dy = take(me);
+  v.dsa = take(mi); v.dza = take(mi); v.r1 = take(n); v.r2 = take(mi); v.r3 = take(mi); v.r4 = take(me);
+  v.u = take(n); v.t = take(me);
+  v.scal = take(8);
+  v.ctl = reinterpret_cast<int*>(take(kCtlWords / 2));
+  return v;
+}
+
+// --------------------------------------------------------------------------------------------
+// Stage: load and sanitise the problem (qpax/pdip.py:166-178 inf rows, :223 symmetrisation).
+// --------------------------------------------------------------------------------------------
+template <int NT, typename TS>
+__device__ void stage_load(const View& v, const TS* Q, const TS* q, const TS* A, const TS* b,
+                           const TS* G, const TS* h) {
+  const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    v.Q[e] = 0.5 * ((double)Q[e] + (double)Q[j * n + i]);
+  }
+  for (int k = tid; k < mi; k += NT) {
+    const double hk = (double)h[k];
+    const bool inf = isinf(hk);
+    v.h[k] = inf ? 0.0 : hk;
+    v.hinf[k] = inf ? 1.0 : 0.0;
+  }
+  for (int i = tid; i < n; i += NT) v.q[i] = (double)q[i];
+  for (int c = tid; c < me; c += NT) v.b[c] = (double)b[c];
+  for (int e = tid; e < me * n; e += NT) v.A[e] = (double)A[e];
+  for (int e = tid; e < mi * n; e += NT) {
+    const int k = e / n;
+    // diag(1 * ~mask) @ G: a masked row becomes 0 * G (pdip.py:176)
+    v.G[e] = isinf((double)h[k]) ? 0.0 * (double)G[e] : (double)G[e];
+  }
+  if (tid < kCtlWords) v.ctl[tid] = 0;
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------------------------
+// Cholesky of a symmetric matrix held in the lower triangle of M (m x m, row-major, leading
+// dimension m).  Overwrites the lower triangle with L.  diag0/rcp receive the original diagonal
+// and 1/L_jj.  A pivot that is not above kPivotRel times its original diagonal entry (or is
+// NaN) is replaced by kHugePivot, which zeroes that component of later solves.
+// --------------------------------------------------------------------------------------------
+template <int NT>
+__device__ void block_cholesky(double* M, int m, double* diag0, double* rcp) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m; i += NT) diag0[i] = M[i * m + i];
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {
+    const double d = M[j * m + j];
+    const bool ok = d > kPivotRel * fabs(diag0[j]);
+    const double ljj = ok ? sqrt(d) : kHugePivot;
+    const double inv = 1.0 / ljj;
+    for (int i = j + 1 + tid; i < m; i += NT) M[i * m + j] *= inv;
+    __syncthreads();
+    if (tid == 0) { M[j * m + j] = ljj; rcp[j] = inv; }
+    const int r = m - j - 1;  // trailing block is r x r, rows/cols j+1..m-1
+    for (int e = tid; e < r * r; e += NT) {
+      const int a = e / r, c = e - a * r;
+      if (c <= a) {
+        const int i = j + 1 + a, k = j + 1 + c;
+        M[i * m + k] -= M[i * m + j] * M[k * m + j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Forward substitution L y = b for one right-hand side; b is overwritten with y.
+template <int NT>
+__device__ void block_fsub(const double* L, const double* rcp, int m, double* b) {
+  const int tid = threadIdx.x;
+  for (int j = 0; j < m; ++j) {
+    const double yj = b[j] * rcp[j];
+    __syncthreads();
+    if (tid == 0) b[j] = yj;
+    for (int i = j + 1 + tid; i < m; i += NT) b[i] -= L[i * m + j] * yj;
+    __syncthreads();
+  }
+}
+
+// Back substitution L^T x = b; b is overwritten with x.
+template <int NT>
+__device__ void block_bsub_t(const double* L, const double* rcp, int m, double* b) {
+  const int tid = threadIdx.x;
+  for (int j = m - 1; j >= 0; --j) {
+    const double xj = b[j] * rcp[j];
+    __syncthreads();
+    if (tid == 0) b[j] = xj;
+    for (int i = tid; i < j; i += NT) b[i] -= L[j * m + i] * xj;
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// Stage: factorisation (qpax/pdip.py:38-46).  D = z/s; H = Q + G^T D G; H = L L^T;
+// W = L^{-1} A^T; F = W^T W = A H^{-1} A^T; F = Lf Lf^T.
+// With unit_d the scaling is D = 1 (the initialisation, pdip.py:17).
+// --------------------------------------------------------------------------------------------
+template <int NT>
+__device__ void stage_factor(const View& v, bool unit_d) {
+  const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  for (int k = tid; k < mi; k += NT) v.D[k] = unit_d ? 1.0 : v.z[k] / v.s[k];
+  __syncthreads();
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    if (j > i) continue;
+    double acc = 0.0;
+    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * (v.D[k] * v.G[k * n + j]);
+    v.H[e] = v.Q[e] + acc;
+  }
+  __syncthreads();
+  block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
+  if (me == 0) return;
+  // W = L^{-1} A^T, all me columns at once: column-oriented forward substitution.
+  for (int e = tid; e < n * me; e += NT) {
+    const int i = e / me, c = e - i * me;
+    v.W[e] = v.A[c * n + i];
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    // row j of W is final once rows < j have been eliminated; scale it after its last use
+    const double rj = v.hrcp[j];
+    for (int e = tid; e < (n - j - 1) * me; e += NT) {
+      const int a = e / me, c = e - a * me, i = j + 1 + a;
+      v.W[i * me + c] -= v.H[i * n + j] * (v.W[j * me + c] * rj);
+    }
+    __syncthreads();
+    for (int c = tid; c < me; c += NT) v.W[j * me + c] *= rj;
+    __syncthreads();
+  }
+  for (int e = tid; e < me * me; e += NT) {
+    const int c = e / me, d = e - c * me;
+    if (d > c) continue;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += v.W[i * me + c] * v.W[i * me + d];
+    v.F[e] = acc;
+  }
+  __syncthreads();
+  block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
+}
+
+// Core of the reduced solve: given p1 (in v.u, length n) and v4, produce dy (v.dy) and dx.
+// dy = F^{-1}(A H^{-1} p1 - v4), dx = H^{-1}(p1 - A^T dy) with A H^{-1} = W^T L^{-1}.
+template <int NT>
+__device__ void reduced_solve(const View& v, const double* v4, double* dx) {
+  const int n = v.n, me = v.me, tid = threadIdx.x;
+  block_fsub<NT>(v.H, v.hrcp, n, v.u);  // u = L^{-1} p1
+  if (me > 0) {
+    for (int c = tid; c < me; c += NT) {
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) acc += v.W[i * me + c] * v.u[i];
+      v.t[c] = acc - v4[c];
+    }
+    __syncthreads();
+    block_fsub<NT>(v.F, v.frcp, me, v.t);
+    block_bsub_t<NT>(v.F, v.frcp, me, v.t);
+    for (int c = tid; c < me; c += NT) v.dy[c] = v.t[c];
+    for (int i = tid; i < n; i += NT) {
+      double acc = 0.0;
+      for (int c = 0; c < me; ++c) acc += v.W[i * me + c] * v.t[c];
+      v.u[i] -= acc;
+    }
+    __syncthreads();
+  }
+  block_bsub_t<NT>(v.H, v.hrcp, n, v.u);
+  for (int i = tid; i < n; i += NT) dx[i] = v.u[i];
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------------------------
+// Stage: solve the KKT system for right-hand side (v1, v2, v3, v4) (qpax/pdip.py:49-63).
+// The right-hand sides are passed negated-or-not through sign flags so callers can hand over the
+// residual vectors directly: v1 = sg1*a1, v2 = sg2*a2 (+ c2 if non-null), v3 = sg3*a3,
+// v4 = sg4*a4.  Writes dx, ds, dz (into the given arrays) and dy (v.dy).
+// --------------------------------------------------------------------------------------------
+template <int NT>
+__device__ void stage_solve(const View& v, const double* a1, const double* a2, const double* a3,
+                            const double* a4, double sg, double* dx, double* ds, double* dz) {
+  const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  // r2' = v3 - v2/z, stored into ds temporarily; p1 = v1 + G^T (D r2')
+  for (int k = tid; k < mi; k += NT) ds[k] = sg * a3[k] - (sg * a2[k]) / v.z[k];
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    double acc = 0.0;
+    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * (v.D[k] * ds[k]);
+    v.u[i] = sg * a1[i] + acc;
+  }
+  // v4 = sg * a4 -> handled by scaling t; write it into dy scratch first
+  for (int c = tid; c < me; c += NT) v.dy[c] = sg * a4[c];
+  __syncthreads();
+  reduced_solve<NT>(v, v.dy, dx);
+  for (int k = tid; k < mi; k += NT) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * dx[j];
+    const double dsk = sg * a3[k] - acc;
+    ds[k] = dsk;
+    dz[k] = (sg * a2[k] - v.z[k] * dsk) / v.s[k];
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------------------------
+// Stage: residuals of the (relaxed) KKT system and the entry convergence flag
+// (qpax/pdip.py:93-100; pdip_relaxed.py:16-23 with kappa).  Returns the flag.
+// --------------------------------------------------------------------------------------------
+template <int NT>
+__device__ int stage_residuals(const View& v, double kappa, double tol, double* red) {
+  const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  double m = 0.0;
+  for (int i = tid; i < n; i += NT) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
+    acc += v.q[i];
+    for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
+    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * v.z[k];
+    v.r1[i] = acc;
+    m = nmax(m, fabs(acc));
+  }
+  for (int k = tid; k < mi; k += NT) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * v.x[j];
+    const double r3 = acc + v.s[k] - v.h[k];
+    const double r2 = v.s[k] * v.z[k] - kappa;
+    v.r2[k] = r2;
+    v.r3[k] = r3;
+    m = nmax(m, nmax(fabs(r2), fabs(r3)));
+  }
+  for (int c = tid; c < me; c += NT) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += v.A[c * n + j] * v.x[j];
+    const double r4 = acc - v.b[c];
+    v.r4[c] = r4;
+    m = nmax(m, fabs(r4));
+  }
+  m = block_reduce<NT>(m, red, OpMax());
+  __syncthreads();
+  return (m < tol) ? 1 : 0;
+}
+
+// min(1, min_i{-x_i/dx_i : dx_i < 0}) over the block (qpax/pdip.py:66-73).
+template <int NT>
+__device__ double block_ort(const double* x, const double* dx, int m, double* red) {
+  double a = 1.0;
+  for (int k = threadIdx.x; k < m; k += NT) {
+    const double d = dx[k];
+    if (d < 0) a = nmin(a, -x[k] / d);
+  }
+  return block_reduce<NT>(a, red, OpMin());
+}
+
+// Stage: Mehrotra centering (qpax/pdip.py:76-84, :107-108).  Updates r2 in place:
+// r2 <- r2 - (sigma mu - ds_a dz_a).
+template <int NT>
+__device__ void stage_center(const View& v, double* red) {
+  const int mi = v.mi, tid = threadIdx.x;
+  double sz = 0.0;
+  for (int k = tid; k < mi; k += NT) sz += v.s[k] * v.z[k];
+  sz = block_reduce<NT>(sz, red, OpSum());
+  const double mu = sz / mi;
+  const double a = nmin(block_ort<NT>(v.s, v.dsa, mi, red), block_ort<NT>(v.z, v.dza, mi, red));
+  double g = 0.0;
+  for (int k = tid; k < mi; k += NT) g += (v.s[k] + a * v.dsa[k]) * (v.z[k] + a * v.dza[k]);
+  g = block_reduce<NT>(g, red, OpSum());
+  const double ratio = g / sz;
+  const double sm = ratio * ratio * ratio * mu;
+  for (int k = tid; k < mi; k += NT) v.r2[k] = v.r2[k] - (sm - v.dsa[k] * v.dza[k]);
+  __syncthreads();
+}
+
+// Stage: line search and update (qpax/pdip.py:112-117 for the predictor-corrector step,
+// pdip_relaxed.py:30-35 for the relaxed Newton step).
+template <int NT>
+__device__ void stage_step(const View& v, bool relax_rule, double* red) {
+  const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  const double as = block_ort<NT>(v.s, v.ds, mi, red);
+  const double az = block_ort<NT>(v.z, v.dz, mi, red);
+  const double alpha = relax_rule ? 0.99 * nmin(nmin(1.0, 0.99 * as), 0.99 * az) : 0.99 * nmin(as, az);
+  for (int i = tid; i < n; i += NT) v.x[i] += alpha * v.dx[i];
+  for (int k = tid; k < mi; k += NT) { v.s[k] += alpha * v.ds[k]; v.z[k] += alpha * v.dz[k]; }
+  for (int c = tid; c < me; c += NT) v.y[c] += alpha * v.dy[c];
+  __syncthreads();
+}
+
+// Stage: initialisation (qpax/pdip.py:15-35).  Factor with D = 1, solve the reduced system with
+// p1 = -q + G^T h and v4 = b, then shift s and z into the positive orthant.
+template <int NT>
+__device__ void stage_init(const View& v, double* red) {
+  const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  stage_factor<NT>(v, true);
+  for (int i = tid; i < n; i += NT) {
+    double acc = -v.q[i];
+    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * v.h[k];
+    v.u[i] = acc;
+  }
+  __syncthreads();
+  reduced_solve<NT>(v, v.b, v.x);
+  for (int c = tid; c < me; c += NT) v.y[c] = v.dy[c];
+  double zmax = -INFINITY, zmin = INFINITY;
+  for (int k = tid; k < mi; k += NT) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * v.x[j];
+    const double zk = acc - v.h[k];
+    v.z[k] = zk;
+    zmax = nmax(zmax, zk);
+    zmin = nmin(zmin, zk);
+  }
+  zmax = block_reduce<NT>(zmax, red, OpMax());
+  zmin = block_reduce<NT>(zmin, red, OpMin());
+  const double alpha_p = zmax, alpha_d = -zmin;
+  for (int k = tid; k < mi; k += NT) {
+    const double zk = v.z[k];
+    v.s[k] = (alpha_p < 0) ? -zk : -zk + (1 + alpha_p);
+    v.z[k] = (alpha_d >= 0) ? zk + (1 + alpha_d) : zk;
+  }
+  __syncthreads();
+}
+
+// One predictor-corrector iteration (qpax/pdip.py:87-119) on an active problem.
+template <int NT>
+__device__ void pc_iteration(const View& v, double tol, double* red) {
+  const int conv = stage_residuals<NT>(v, 0.0, tol, red);
+  stage_factor<NT>(v, false);
+  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.dsa, v.dza);
+  stage_center<NT>(v, red);
+  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
+  stage_step<NT>(v, false, red);
+  if (threadIdx.x == 0) { v.ctl[kConv] = conv; v.ctl[kIter] += 1; }
+  __syncthreads();
+}
+
+// One relaxed Newton iteration (qpax/pdip_relaxed.py:7-52).
+template <int NT>
+__device__ void relax_iteration(const View& v, double tol, double kappa, double* red) {
+  const int conv = stage_residuals<NT>(v, kappa, tol, red);
+  stage_factor<NT>(v, false);
+  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
+  stage_step<NT>(v, true, red);
+  if (threadIdx.x == 0) { v.ctl[kRelaxConv] = conv; v.ctl[kRelaxIter] += 1; }
+  __syncthreads();
+}
+
+}  // namespace qpx

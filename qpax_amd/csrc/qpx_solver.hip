@@ -1,0 +1,475 @@
+// Kernels of the generic (any size) solver path and the C ABI of libqpx.
+//
+// One thread block owns one QP.  The plain mode launches one kernel per IPM stage per iteration
+// over the whole padded batch and reads the number of still-active problems back to the host
+// after every iteration; the fused mode runs each phase (solve + relax, backward) as a single
+// launch in which every block iterates its own problem to completion.  Both modes call the same
+// device stages in the same order, so they produce bit-identical results.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <type_traits>
+
+#include "../../include/qpx.h"
+#include "qpx_core.hpp"
+
+namespace qpx {
+
+template <typename TS>
+struct Args {
+  int64_t B;
+  int n, me, mi;
+  const TS *Q, *q, *A, *b, *G, *h;
+  TS *x, *s, *z, *y;
+  int *conv, *iters;
+  TS *xr, *sr, *zr, *yr;
+  int *rconv, *riters;
+  const TS *xr_in, *sr_in, *zr_in, *yr_in, *g;
+  TS *dQ, *dq, *dA, *db, *dG, *dh;
+  double* ws;
+  int64_t stride;  // float64 words per problem
+  unsigned* counter;
+  double tol, kappa;
+};
+
+template <typename TS>
+__device__ __forceinline__ View view_of(const Args<TS>& a, int64_t p) {
+  return carve(a.ws + p * a.stride, a.n, a.me, a.mi);
+}
+
+// ------------------------------------------------------------------------------------------
+// setup: load + sanitise + initialise (qpx/pdip.py:204-225)
+// ------------------------------------------------------------------------------------------
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_setup(Args<TS> a) {
+  __shared__ double red[NT / 64];
+  const int64_t p = blockIdx.x;
+  const View v = view_of(a, p);
+  const int64_t n = a.n, me = a.me, mi = a.mi;
+  stage_load<NT, TS>(v, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me,
+                     a.G + p * mi * n, a.h + p * mi);
+  stage_init<NT>(v, red);
+}
+
+// Active predicate of jax.lax.while_loop(iter < 30 and converged == 0) (pdip.py:228-232).
+__device__ __forceinline__ bool is_active(const View& v, bool relax) {
+  const int c = v.ctl[relax ? kRelaxConv : kConv], it = v.ctl[relax ? kRelaxIter : kIter];
+  return it < kMaxIters && c == 0;
+}
+
+// ---------------------------------- plain-mode stage kernels --------------------------------
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_resid(Args<TS> a, int relax) {
+  __shared__ double red[NT / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;
+  const View v = view_of(a, blockIdx.x);
+  const bool act = is_active(v, relax);
+  __syncthreads();
+  if (threadIdx.x == 0) v.ctl[kActive] = act ? 1 : 0;
+  if (!act) return;
+  const int conv = stage_residuals<NT>(v, relax ? a.kappa : 0.0, a.tol, red);
+  if (threadIdx.x == 0) v.ctl[5] = conv;  // entry flag, committed by k_step
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_factor(Args<TS> a) {
+  const View v = view_of(a, blockIdx.x);
+  if (!v.ctl[kActive]) return;
+  stage_factor<NT>(v, false);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_solve(Args<TS> a, int affine) {
+  const View v = view_of(a, blockIdx.x);
+  if (!v.ctl[kActive]) return;
+  if (affine) stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.dsa, v.dza);
+  else stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_center(Args<TS> a) {
+  __shared__ double red[NT / 64];
+  const View v = view_of(a, blockIdx.x);
+  if (!v.ctl[kActive]) return;
+  stage_center<NT>(v, red);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_step(Args<TS> a, int relax) {
+  __shared__ double red[NT / 64];
+  const View v = view_of(a, blockIdx.x);
+  if (!v.ctl[kActive]) return;
+  stage_step<NT>(v, relax != 0, red);
+  if (threadIdx.x == 0) {
+    v.ctl[relax ? kRelaxConv : kConv] = v.ctl[5];
+    v.ctl[relax ? kRelaxIter : kIter] += 1;
+    if (is_active(v, relax)) atomicAdd(a.counter, 1u);
+  }
+}
+
+// Post-processing of solve_qp (pdip.py:243-251) and start of the relaxation.
+template <int NT, typename TS>
+__device__ void finish_solve(const Args<TS>& a, const View& v, int64_t p) {
+  const int n = a.n, me = a.me, mi = a.mi, tid = threadIdx.x;
+  for (int k = tid; k < mi; k += NT) {
+    if (v.hinf[k] != 0.0) { v.z[k] = 0.0; v.s[k] = INFINITY; }
+    a.s[p * mi + k] = (TS)v.s[k];
+    a.z[p * mi + k] = (TS)v.z[k];
+  }
+  for (int i = tid; i < n; i += NT) a.x[p * n + i] = (TS)v.x[i];
+  for (int c = tid; c < me; c += NT) a.y[p * me + c] = (TS)v.y[c];
+  __syncthreads();
+  if (tid == 0) {
+    a.conv[p] = v.ctl[kConv];
+    a.iters[p] = v.ctl[kIter];
+    v.ctl[kRelaxConv] = 0;
+    v.ctl[kRelaxIter] = 0;
+  }
+  __syncthreads();
+}
+
+template <int NT, typename TS>
+__device__ void finish_relax(const Args<TS>& a, const View& v, int64_t p) {
+  const int n = a.n, me = a.me, mi = a.mi, tid = threadIdx.x;
+  for (int k = tid; k < mi; k += NT) { a.sr[p * mi + k] = (TS)v.s[k]; a.zr[p * mi + k] = (TS)v.z[k]; }
+  for (int i = tid; i < n; i += NT) a.xr[p * n + i] = (TS)v.x[i];
+  for (int c = tid; c < me; c += NT) a.yr[p * me + c] = (TS)v.y[c];
+  if (tid == 0) { a.rconv[p] = v.ctl[kRelaxConv]; a.riters[p] = v.ctl[kRelaxIter]; }
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_finish_solve(Args<TS> a) {
+  finish_solve<NT, TS>(a, view_of(a, blockIdx.x), blockIdx.x);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_finish_relax(Args<TS> a) {
+  finish_relax<NT, TS>(a, view_of(a, blockIdx.x), blockIdx.x);
+}
+
+// ------------------------------------ fused forward ----------------------------------------
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_fused_solve(Args<TS> a, int with_relax) {
+  __shared__ double red[NT / 64];
+  const int64_t p = blockIdx.x;
+  const View v = view_of(a, p);
+  const int64_t n = a.n, me = a.me, mi = a.mi;
+  stage_load<NT, TS>(v, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me,
+                     a.G + p * mi * n, a.h + p * mi);
+  stage_init<NT>(v, red);
+  while (is_active(v, false)) {
+    __syncthreads();
+    pc_iteration<NT>(v, a.tol, red);
+  }
+  finish_solve<NT, TS>(a, v, p);
+  if (!with_relax) return;
+  while (is_active(v, true)) {
+    __syncthreads();
+    relax_iteration<NT>(v, a.tol, a.kappa, red);
+  }
+  finish_relax<NT, TS>(a, v, p);
+}
+
+// ------------------------------------ backward ---------------------------------------------
+// diff_qp (qpax/diff_qp.py:20-46): load the relaxed point and the cotangent.
+template <int NT, typename TS>
+__device__ void bwd_load(const Args<TS>& a, const View& v, int64_t p) {
+  const int64_t n = a.n, me = a.me, mi = a.mi;
+  stage_load<NT, TS>(v, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me,
+                     a.G + p * mi * n, a.h + p * mi);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += NT) { v.x[i] = (double)a.xr_in[p * n + i]; v.r1[i] = (double)a.g[p * n + i]; }
+  for (int k = tid; k < mi; k += NT) {
+    v.s[k] = (double)a.sr_in[p * mi + k];
+    v.z[k] = (double)a.zr_in[p * mi + k];
+    v.r2[k] = 0.0;
+    v.r3[k] = 0.0;
+  }
+  for (int c = tid; c < me; c += NT) { v.y[c] = (double)a.yr_in[p * me + c]; v.r4[c] = 0.0; }
+  __syncthreads();
+}
+
+// Solve with rhs (-dl_dx, 0, 0, 0) and form the cotangents (optnet_derivatives, diff_qp.py:8-17).
+template <int NT, typename TS>
+__device__ void bwd_solve_write(const Args<TS>& a, const View& v, int64_t p) {
+  const int n = a.n, me = a.me, mi = a.mi, tid = threadIdx.x;
+  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
+  // dz here is dlam_tilde; dlam = dlam_tilde / lam
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    a.dQ[p * n * n + e] = (TS)(0.5 * (v.dx[i] * v.x[j] + v.x[i] * v.dx[j]));
+  }
+  for (int i = tid; i < n; i += NT) a.dq[p * n + i] = (TS)v.dx[i];
+  for (int e = tid; e < me * n; e += NT) {
+    const int c = e / n, j = e - c * n;
+    a.dA[(int64_t)p * me * n + e] = (TS)(v.dy[c] * v.x[j] + v.y[c] * v.dx[j]);
+  }
+  for (int c = tid; c < me; c += NT) a.db[p * me + c] = (TS)(-v.dy[c]);
+  for (int e = tid; e < mi * n; e += NT) {
+    const int k = e / n, j = e - k * n;
+    const double lam = v.z[k], dlam = v.dz[k] / lam;
+    a.dG[(int64_t)p * mi * n + e] = (TS)(lam * (dlam * v.x[j] + lam * v.dx[j]));
+  }
+  for (int k = tid; k < mi; k += NT) {
+    const double lam = v.z[k], dlam = v.dz[k] / lam;
+    a.dh[p * mi + k] = (TS)(-lam * dlam);
+  }
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_bwd_load(Args<TS> a) {
+  bwd_load<NT, TS>(a, view_of(a, blockIdx.x), blockIdx.x);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_bwd_factor(Args<TS> a) {
+  stage_factor<NT>(view_of(a, blockIdx.x), false);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_bwd_solve(Args<TS> a) {
+  bwd_solve_write<NT, TS>(a, view_of(a, blockIdx.x), blockIdx.x);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_fused_bwd(Args<TS> a) {
+  const int64_t p = blockIdx.x;
+  const View v = view_of(a, p);
+  bwd_load<NT, TS>(a, v, p);
+  stage_factor<NT>(v, false);
+  bwd_solve_write<NT, TS>(a, v, p);
+}
+
+}  // namespace qpx
+
+// ===================================== host side =============================================
+namespace {
+
+thread_local std::string g_err;
+
+qpx_status fail(qpx_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define QPX_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return fail(QPX_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr size_t kCtlBytes = 4096;  // head of the workspace: counters
+
+int64_t stride_words(int n, int me, int mi) { return qpx::view_words(n, me, mi); }
+
+size_t ws_bytes(int64_t B, int n, int me, int mi) {
+  return kCtlBytes + (size_t)B * (size_t)stride_words(n, me, mi) * sizeof(double);
+}
+
+qpx_status check_dims(int32_t dtype, int32_t mode, int64_t B, int32_t n, int32_t me, int32_t mi,
+                      size_t have) {
+  if (dtype != QPX_F32 && dtype != QPX_F64) return fail(QPX_ERR_ARG, "dtype must be QPX_F32 or QPX_F64 (got %d)", dtype);
+  if (mode != QPX_PLAIN && mode != QPX_FUSED) return fail(QPX_ERR_ARG, "mode must be QPX_PLAIN or QPX_FUSED (got %d)", mode);
+  if (B < 0 || n < 1 || me < 0 || mi < 0) return fail(QPX_ERR_ARG, "invalid sizes batch=%lld n=%d m_eq=%d m_ineq=%d", (long long)B, n, me, mi);
+  if (mi < 1) return fail(QPX_ERR_ARG, "m_ineq must be >= 1 for the interior-point path (got %d)", mi);
+  if (B > 0x7fffffffLL) return fail(QPX_ERR_UNSUPPORTED, "batch %lld exceeds 2^31-1", (long long)B);
+  const size_t need = ws_bytes(B, n, me, mi);
+  if (have < need) return fail(QPX_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu required", have, need);
+  return QPX_OK;
+}
+
+template <typename TS>
+qpx::Args<TS> make_args(int64_t B, int n, int me, int mi, void* ws) {
+  qpx::Args<TS> a;
+  std::memset(&a, 0, sizeof a);
+  a.B = B; a.n = n; a.me = me; a.mi = mi;
+  a.counter = reinterpret_cast<unsigned*>(ws);
+  a.ws = reinterpret_cast<double*>(static_cast<char*>(ws) + kCtlBytes);
+  a.stride = stride_words(n, me, mi);
+  return a;
+}
+
+int* pinned_counter() {
+  thread_local int* p = nullptr;
+  if (!p) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(int), hipHostMallocDefault) != hipSuccess) p = nullptr;
+  }
+  return p;
+}
+
+template <int NT, typename TS>
+qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
+  const dim3 grid((unsigned)a.B), block(NT);
+  if (a.B == 0) return QPX_OK;
+  if (mode == QPX_FUSED) {
+    hipLaunchKernelGGL((qpx::k_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
+    QPX_HIP(hipGetLastError());
+    return QPX_OK;
+  }
+  int* host = pinned_counter();
+  if (!host) return fail(QPX_ERR_HIP, "hipHostMalloc of the plain-mode counter failed");
+  hipLaunchKernelGGL((qpx::k_setup<NT, TS>), grid, block, 0, st, a);
+  QPX_HIP(hipGetLastError());
+  for (int phase = 0; phase < (relax ? 2 : 1); ++phase) {
+    const int rl = phase;
+    for (int it = 0; it < qpx::kMaxIters; ++it) {
+      hipLaunchKernelGGL((qpx::k_resid<NT, TS>), grid, block, 0, st, a, rl);
+      hipLaunchKernelGGL((qpx::k_factor<NT, TS>), grid, block, 0, st, a);
+      if (!rl) {
+        hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, 0, st, a, 1);
+        hipLaunchKernelGGL((qpx::k_center<NT, TS>), grid, block, 0, st, a);
+      }
+      hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, 0, st, a, 0);
+      hipLaunchKernelGGL((qpx::k_step<NT, TS>), grid, block, 0, st, a, rl);
+      QPX_HIP(hipGetLastError());
+      QPX_HIP(hipMemcpyAsync(host, a.counter, sizeof(int), hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipStreamSynchronize(st));
+      if (*host == 0) break;
+    }
+    if (!rl) hipLaunchKernelGGL((qpx::k_finish_solve<NT, TS>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((qpx::k_finish_relax<NT, TS>), grid, block, 0, st, a);
+    QPX_HIP(hipGetLastError());
+  }
+  return QPX_OK;
+}
+
+template <int NT, typename TS>
+qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
+  const dim3 grid((unsigned)a.B), block(NT);
+  if (a.B == 0) return QPX_OK;
+  if (mode == QPX_FUSED) {
+    hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, 0, st, a);
+  } else {
+    hipLaunchKernelGGL((qpx::k_bwd_load<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_bwd_factor<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_bwd_solve<NT, TS>), grid, block, 0, st, a);
+  }
+  QPX_HIP(hipGetLastError());
+  return QPX_OK;
+}
+
+// Threads per block by problem size: one wavefront for small problems, more for large ones.
+template <typename F>
+qpx_status dispatch_nt(int n, int mi, F&& f) {
+  if (n <= 32 && mi <= 128) return f(std::integral_constant<int, 64>());
+  if (n <= 96) return f(std::integral_constant<int, 128>());
+  return f(std::integral_constant<int, 256>());
+}
+
+template <typename TS>
+qpx_status forward_impl(int mode, int64_t B, int n, int me, int mi, const void* Q, const void* q,
+                        const void* A, const void* b, const void* G, const void* h, double tol,
+                        double kappa, bool relax, void* x, void* s, void* z, void* y, int* conv,
+                        int* iters, void* xr, void* sr, void* zr, void* yr, int* rconv,
+                        int* riters, void* ws, void* stream) {
+  auto a = make_args<TS>(B, n, me, mi, ws);
+  a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+  a.G = (const TS*)G; a.h = (const TS*)h;
+  a.x = (TS*)x; a.s = (TS*)s; a.z = (TS*)z; a.y = (TS*)y; a.conv = conv; a.iters = iters;
+  a.xr = (TS*)xr; a.sr = (TS*)sr; a.zr = (TS*)zr; a.yr = (TS*)yr; a.rconv = rconv; a.riters = riters;
+  a.tol = tol; a.kappa = kappa;
+  return dispatch_nt(n, mi, [&](auto nt) { return run_forward<decltype(nt)::value, TS>(a, mode, relax, (hipStream_t)stream); });
+}
+
+template <typename TS>
+qpx_status backward_impl(int mode, int64_t B, int n, int me, int mi, const void* Q, const void* q,
+                         const void* A, const void* b, const void* G, const void* h,
+                         const void* xr, const void* sr, const void* zr, const void* yr,
+                         const void* g, void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
+                         void* ws, void* stream) {
+  auto a = make_args<TS>(B, n, me, mi, ws);
+  a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+  a.G = (const TS*)G; a.h = (const TS*)h;
+  a.xr_in = (const TS*)xr; a.sr_in = (const TS*)sr; a.zr_in = (const TS*)zr; a.yr_in = (const TS*)yr;
+  a.g = (const TS*)g;
+  a.dQ = (TS*)dQ; a.dq = (TS*)dq; a.dA = (TS*)dA; a.db = (TS*)db; a.dG = (TS*)dG; a.dh = (TS*)dh;
+  return dispatch_nt(n, mi, [&](auto nt) { return run_backward<decltype(nt)::value, TS>(a, mode, (hipStream_t)stream); });
+}
+
+bool any_null(std::initializer_list<const void*> ps) {
+  for (auto p : ps) if (!p) return true;
+  return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+QPX_API int32_t qpx_version(void) { return 100; }
+
+QPX_API const char* qpx_last_error(void) { return g_err.c_str(); }
+
+QPX_API size_t qpx_workspace_size(int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq) {
+  if (batch < 0 || n < 1 || m_eq < 0 || m_ineq < 0) return 0;
+  return ws_bytes(batch, n, m_eq, m_ineq);
+}
+
+QPX_API qpx_status qpx_solve_qp(int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq,
+                                int32_t m_ineq, const void* Q, const void* q, const void* A,
+                                const void* b, const void* G, const void* h, double solver_tol,
+                                void* x, void* s, void* z, void* y, int32_t* converged,
+                                int32_t* iters, void* workspace, size_t workspace_bytes,
+                                void* stream) {
+  g_err.clear();
+  qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, x, s, z, converged, iters, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (dtype == QPX_F32)
+    return forward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, solver_tol, 0.0, false, x, s, z, y,
+                               converged, iters, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, workspace, stream);
+  return forward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, solver_tol, 0.0, false, x, s, z, y,
+                              converged, iters, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, workspace, stream);
+}
+
+QPX_API qpx_status qpx_solve_qp_relaxed(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
+                                        int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
+                                        const void* A, const void* b, const void* G, const void* h,
+                                        double solver_tol, double target_kappa, void* x, void* s,
+                                        void* z, void* y, int32_t* converged, int32_t* iters,
+                                        void* xr, void* sr, void* zr, void* yr,
+                                        int32_t* relax_converged, int32_t* relax_iters,
+                                        void* workspace, size_t workspace_bytes, void* stream) {
+  g_err.clear();
+  qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, x, s, z, converged, iters, xr, sr, zr, relax_converged, relax_iters, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y, yr})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (dtype == QPX_F32)
+    return forward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, solver_tol, target_kappa, true, x, s, z,
+                               y, converged, iters, xr, sr, zr, yr, relax_converged, relax_iters, workspace, stream);
+  return forward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, solver_tol, target_kappa, true, x, s, z,
+                              y, converged, iters, xr, sr, zr, yr, relax_converged, relax_iters, workspace, stream);
+}
+
+QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq,
+                               int32_t m_ineq, const void* Q, const void* q, const void* A,
+                               const void* b, const void* G, const void* h, const void* xr,
+                               const void* sr, const void* zr, const void* yr, const void* dl_dx,
+                               void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
+                               void* workspace, size_t workspace_bytes, void* stream) {
+  g_err.clear();
+  qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, xr, sr, zr, dl_dx, dQ, dq, dG, dh, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, yr, dA, db})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (dtype == QPX_F32)
+    return backward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, dQ, dq, dA, db,
+                                dG, dh, workspace, stream);
+  return backward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, dQ, dq, dA, db,
+                               dG, dh, workspace, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+"""qpax.solve_qp on the HIP library -- host side of qpax/pdip.py:181-251.
+
+Same arguments and return tuple as the reference: ``solve_qp(Q, q, A, b, G, h, solver_tol=1e-3)
+-> (x, s, z, y, converged, iters)``.  Inputs are torch tensors (or arrays) with any leading batch
+axes; the whole batch is solved by one library call.  Arithmetic is float64 inside the kernels
+for either input dtype; outputs have the input dtype, flags are int32.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from ._tensors import Batch, dtype_code, normalize, ptr, stream_of
+
+
+def _workspace(bt: Batch):
+    nbytes = _lib.load().qpx_workspace_size(bt.B, bt.n, bt.me, bt.mi)
+    return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=bt.device), nbytes
+
+
+def _special(bt: Batch):
+    """Static shape branches of pdip.py:208-217 (no inequality rows)."""
+    from .special import solve_eq_only, solve_unconstrained
+
+    if bt.me == 0:
+        x = solve_unconstrained(bt)
+        y = bt.out(0)
+    else:
+        x, y = solve_eq_only(bt)
+    e = bt.out(0)
+    B = bt.B
+    one = torch.ones(B, dtype=torch.int32, device=bt.device)
+    zero = torch.zeros(B, dtype=torch.int32, device=bt.device)
+    return (bt.unflatten(x, bt.n), bt.unflatten(e, 0), bt.unflatten(e, 0), bt.unflatten(y, bt.me),
+            bt.unflatten(one), bt.unflatten(zero))
+
+
+def solve_qp_batch(bt: Batch, solver_tol=1e-3, mode="fused"):
+    """Forward solve of a normalised batch; returns flat (B, ...) tensors."""
+    lib = _lib.load()
+    x, s, z, y = bt.out(bt.n), bt.out(bt.mi), bt.out(bt.mi), bt.out(bt.me)
+    conv = torch.empty(bt.B, dtype=torch.int32, device=bt.device)
+    iters = torch.empty(bt.B, dtype=torch.int32, device=bt.device)
+    ws, nbytes = _workspace(bt)
+    st = lib.qpx_solve_qp(dtype_code(bt.dtype), _lib.MODES[mode], bt.B, bt.n, bt.me, bt.mi,
+                          *[ptr(bt[k]) for k in bt.names], float(solver_tol),
+                          ptr(x), ptr(s), ptr(z), ptr(y), ptr(conv), ptr(iters),
+                          ptr(ws), nbytes, stream_of(bt.device))
+    _lib.check(st, "qpx_solve_qp")
+    return x, s, z, y, conv, iters
+
+
+def solve_qp_relaxed_batch(bt: Batch, solver_tol=1e-5, target_kappa=1e-3, mode="fused"):
+    """Solve then relax (diff_qp.py:61-71); returns (x, s, z, y, conv, iters, xr, sr, zr, yr,
+    rconv, riters) as flat tensors."""
+    lib = _lib.load()
+    x, s, z, y = bt.out(bt.n), bt.out(bt.mi), bt.out(bt.mi), bt.out(bt.me)
+    xr, sr, zr, yr = bt.out(bt.n), bt.out(bt.mi), bt.out(bt.mi), bt.out(bt.me)
+    flags = torch.empty(4, bt.B, dtype=torch.int32, device=bt.device)
+    ws, nbytes = _workspace(bt)
+    st = lib.qpx_solve_qp_relaxed(dtype_code(bt.dtype), _lib.MODES[mode], bt.B, bt.n, bt.me, bt.mi,
+                                  *[ptr(bt[k]) for k in bt.names], float(solver_tol), float(target_kappa),
+                                  ptr(x), ptr(s), ptr(z), ptr(y), ptr(flags[0]), ptr(flags[1]),
+                                  ptr(xr), ptr(sr), ptr(zr), ptr(yr), ptr(flags[2]), ptr(flags[3]),
+                                  ptr(ws), nbytes, stream_of(bt.device))
+    _lib.check(st, "qpx_solve_qp_relaxed")
+    return x, s, z, y, flags[0], flags[1], xr, sr, zr, yr, flags[2], flags[3]
+
+
+def solve_qp(Q, q, A, b, G, h, solver_tol: float = 1e-3, mode: str = "fused"):
+    """Solve min 1/2 x'Qx + q'x s.t. Ax = b, Gx <= h (qpax/pdip.py:181).
+
+    Returns (x, s, z, y, converged, iters).  ``mode`` selects the fused single-launch path or the
+    plain per-stage path; both return identical results."""
+    bt = normalize(Q, q, A, b, G, h)
+    if bt.mi == 0:
+        return _special(bt)
+    x, s, z, y, conv, iters = solve_qp_batch(bt, solver_tol, mode)
+    return (bt.unflatten(x, bt.n), bt.unflatten(s, bt.mi), bt.unflatten(z, bt.mi), bt.unflatten(y, bt.me),
+            bt.unflatten(conv), bt.unflatten(iters))

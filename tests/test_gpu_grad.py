@@ -1,0 +1,56 @@
+"""GPU parity of solve_qp_primal's forward relaxation and backward against the float64 oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import qp_oracle as O
+from qpax_amd import problems as P
+from tests._util import rel_err, to_np
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+GCASES = [
+    ("c3-mini-f64", lambda: P.random_feasible(32, 50, 10, 100, seed=3), np.float64, 1e-7),
+    ("c3-mini-f32", lambda: P.random_feasible(32, 50, 10, 100, seed=3), np.float32, 1e-3),
+    ("c2-mini-f32", lambda: P.random_feasible(32, 20, 5, 40, seed=2), np.float32, 1e-3),
+    ("planted-f64", lambda: P.planted_kkt(32, 15, 10, 3, seed=3)[:6], np.float64, 1e-7),
+]
+
+
+def _grads(data, dt, mode):
+    import qpax_amd
+
+    ts = [torch.tensor(np.asarray(v, dt), device="cuda", requires_grad=True) for v in data]
+    x = qpax_amd.solve_qp_primal(*ts, solver_tol=1e-5, target_kappa=1e-3, mode=mode)
+    loss = ((x - 1.0) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    return to_np(x), [to_np(t.grad) for t in ts]
+
+
+@pytest.mark.parametrize("name,gen,dt,gtol", GCASES, ids=[c[0] for c in GCASES])
+def test_grads_match_oracle(name, gen, dt, gtol):
+    data = [np.asarray(v, dt) for v in gen()]
+    print(f"[case] {name} shapes={[d.shape for d in data]}", flush=True)
+    x, g = _grads(data, dt, "fused")
+    d64 = [v.astype(np.float64) for v in data]
+    xo, rel, info = O.solve_qp_primal_fwd(*d64, solver_tol=1e-5, target_kappa=1e-3)
+    go = O.solve_qp_primal_bwd(*d64, rel, 2 * (xo - 1))
+    ok = (info[0] == 1) & (info[2] == 1)
+    assert ok.mean() > 0.9
+    for nm, a, b in zip("Q q A b G h".split(), g, go):
+        if b.shape[1] == 0:
+            continue
+        e = rel_err(a[ok], b[ok])
+        print(f"{name} d{nm}: max {e.max():.2e} median {np.median(e):.2e}")
+        assert e.max() < gtol
+
+
+def test_grads_plain_equals_fused():
+    data = [np.asarray(v, np.float32) for v in P.random_feasible(16, 50, 10, 100, seed=7)]
+    xa, ga = _grads(data, np.float32, "fused")
+    xb, gb = _grads(data, np.float32, "plain")
+    np.testing.assert_array_equal(xa, xb)
+    for a, b in zip(ga, gb):
+        np.testing.assert_array_equal(a, b)

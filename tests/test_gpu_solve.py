@@ -1,0 +1,57 @@
+"""GPU parity of the forward solve and the relaxation against the float64 oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import qp_oracle as O
+from qpax_amd import problems as P
+from tests._util import rel_err, to_np
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (name, generator, dtype, tol, x_tol)
+    ("c3-mini-f64", lambda: P.random_feasible(64, 50, 10, 100, seed=3), np.float64, 1e-5, 1e-9),
+    ("c3-mini-f32", lambda: P.random_feasible(64, 50, 10, 100, seed=3), np.float32, 1e-5, 1e-4),
+    ("c2-mini-f64", lambda: P.random_feasible(64, 20, 5, 40, seed=2), np.float64, 1e-3, 1e-9),
+    ("c2-mini-f32", lambda: P.random_feasible(64, 20, 5, 40, seed=2), np.float32, 1e-3, 1e-4),
+    ("c1-nnls-f64", lambda: P.nnls(256, seed=1), np.float64, 1e-10, 1e-9),
+    ("c1-nnls-f32", lambda: P.nnls(256, seed=1), np.float32, 1e-5, 1e-4),
+]
+
+
+def _run(name, gen, dt, tol, mode):
+    import qpax_amd
+
+    data = [np.asarray(v, dt) for v in gen()]
+    print(f"[case] {name} mode={mode} shapes={[d.shape for d in data]}", flush=True)
+    dev = [torch.as_tensor(v, device="cuda") for v in data]
+    out = qpax_amd.solve_qp(*dev, solver_tol=tol, mode=mode)
+    torch.cuda.synchronize()
+    return data, [to_np(t) for t in out]
+
+
+@pytest.mark.parametrize("name,gen,dt,tol,xtol", CASES, ids=[c[0] for c in CASES])
+def test_solve_matches_oracle(name, gen, dt, tol, xtol):
+    data, (x, s, z, y, conv, iters) = _run(name, gen, dt, tol, "fused")
+    ref = O.solve_qp(*[d.astype(np.float64) for d in data], solver_tol=tol)
+    xo, so, zo, yo, co, io = ref
+    assert x.dtype == dt
+    ok = co == 1
+    assert ok.mean() > 0.9
+    np.testing.assert_array_equal(conv, co)
+    assert np.abs(iters - io).max() <= 1
+    ex = rel_err(x[ok], xo[ok])
+    print(f"{name}: x rel err max {ex.max():.2e} median {np.median(ex):.2e}")
+    assert ex.max() < xtol
+    for a, b in ((s, so), (z, zo)):
+        assert rel_err(a[ok], b[ok]).max() < max(xtol, 1e-6) * 10
+
+
+@pytest.mark.parametrize("name,gen,dt,tol,xtol", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_plain_equals_fused(name, gen, dt, tol, xtol):
+    _, a = _run(name, gen, dt, tol, "fused")
+    _, b = _run(name, gen, dt, tol, "plain")
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
