@@ -1,0 +1,232 @@
+"""Benchmark: batched QP forward+backward throughput on one MI355X.
+
+Workload (the headline metric's configuration): 4096 random feasible dense QPs, n=50, m_eq=10,
+m_ineq=100, float32 inputs/outputs; one step = qpax_amd.solve_qp_primal forward (solve + kappa
+relaxation) and backward of loss = ||x - 1||^2 at solver_tol=1e-5, target_kappa=1e-3, i.e. the
+gradients w.r.t. Q, q, A, b, G, h.  The fused path (one launch for solve+relax, one for the
+backward) is timed for --steps steps; the plain per-stage path is timed on --plain-steps steps for
+the fused/plain ratio.  Inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line (rank 0).  Extra objects: "roofline" for the dominant kernel (fused solve +
+relax) from HIP events on the launch stream, and "cpu_baseline" (the float64 numpy oracle on a
+bounded sample of the same workload, single-threaded).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N, ME, MI, BATCH, SEED = 50, 10, 100, 4096, 3
+TOL, KAPPA = 1e-5, 1e-3
+PEAK_F64_TFLOPS = 78.6  # MI355X float64 (vector and matrix) dense peak
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_per_qp(n, me, mi, it_fwd, it_rx, bwd=True):
+    """Algorithmic flops of one QP (multiply-add = 2; dense, non-symmetric G^T D G; Cholesky)."""
+    f_res = 2 * n * n + 4 * n * mi + 4 * n * me
+    f_fac = 2 * mi * n * n + n ** 3 / 3 + 2 * n * n * me + 2 * n * me * me + me ** 3 / 3
+    f_sol = 4 * n * mi + 4 * n * n + 4 * n * me + 2 * me * me
+    f_pc = f_res + f_fac + 2 * f_sol
+    f_rx = f_res + f_fac + f_sol
+    f_init = f_fac + f_sol + 2 * n * mi
+    f_bwd = f_fac + f_sol + 3 * n * n + 2 * n * me + 3 * n * mi
+    fwd = f_init + it_fwd * f_pc + it_rx * f_rx
+    return fwd, (f_bwd if bwd else 0.0)
+
+
+def bytes_per_qp(n, me, mi, isz=4):
+    inputs = n * n + n + me * n + me + mi * n + mi
+    fwd_out = 2 * (n + 2 * mi + me) * isz + 16  # x,s,z,y and xr,sr,zr,yr + flags
+    return inputs * isz, fwd_out, (n + inputs) * isz
+
+
+def make_inputs(torch, dev):
+    from qpax_amd import problems as P
+
+    data = [np.asarray(v, np.float32) for v in P.random_feasible(BATCH, N, ME, MI, seed=SEED)]
+    return data, [torch.tensor(v, device=dev, requires_grad=True) for v in data]
+
+
+def fwd_bwd(qpax_amd, ts, mode):
+    for t in ts:
+        t.grad = None
+    x = qpax_amd.solve_qp_primal(*ts, solver_tol=TOL, target_kappa=KAPPA, mode=mode)
+    loss = ((x - 1.0) ** 2).sum()
+    loss.backward()
+    return x
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / max(steps, 1)
+
+
+def kernel_times(torch, qpax_amd, ts, reps):
+    """Average duration of the fused solve+relax launch and of the fused backward launch, from HIP
+    events recorded on the stream the library launches on (torch's current stream)."""
+    from qpax_amd._tensors import normalize
+    from qpax_amd.diff_qp import diff_qp_batch
+    from qpax_amd.pdip import solve_qp_relaxed_batch
+
+    bt = normalize(*[t.detach() for t in ts])
+    st = torch.cuda.current_stream()
+    f_ms, b_ms = [], []
+    out = None
+    for r in range(reps + 1):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(st)
+        out = solve_qp_relaxed_batch(bt, TOL, KAPPA, "fused")
+        e1.record(st)
+        g = 2.0 * (out[0] - 1.0)
+        e1b = torch.cuda.Event(enable_timing=True)
+        e1b.record(st)
+        diff_qp_batch(bt, out[6:10], g, "fused")
+        e2.record(st)
+        torch.cuda.synchronize()
+        if r > 0:
+            f_ms.append(e0.elapsed_time(e1))
+            b_ms.append(e1b.elapsed_time(e2))
+    return float(np.mean(f_ms)), float(np.mean(b_ms)), out
+
+
+def cpu_baseline(data, sample, xg_gpu):
+    """float64 numpy oracle, one thread, on the first `sample` problems: fwd (solve + relax) + bwd."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import qp_oracle as O
+
+    d64 = [v[:sample].astype(np.float64) for v in data]
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        xo, rel, info = O.solve_qp_primal_fwd(*d64, solver_tol=TOL, target_kappa=KAPPA)
+        go = O.solve_qp_primal_bwd(*d64, rel, 2 * (xo - 1))
+        dt = time.perf_counter() - t0
+    return sample / dt, xo, go, info
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="only run a few fused steps (for rocprofv3)")
+    ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a plain device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
+        sys.exit(2)
+
+    import torch
+
+    import qpax_amd
+
+    dev = torch.device("cuda", 0)
+    if args.copy_calib:
+        nb = args.copy_calib << 20
+        a = torch.empty(nb // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": nb}))
+        return
+
+    data, ts = make_inputs(torch, dev)
+    if args.profile:
+        for _ in range(max(args.steps, 1)):
+            fwd_bwd(qpax_amd, ts, "fused")
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_steps": args.steps}), flush=True)
+        return
+
+    ms_fused = 1e3 * time_steps(torch, lambda: fwd_bwd(qpax_amd, ts, "fused"), args.steps, args.warmup)
+    print(f"# fused fwd+bwd: {ms_fused:.3f} ms/step", file=sys.stderr, flush=True)
+    ms_plain = 1e3 * time_steps(torch, lambda: fwd_bwd(qpax_amd, ts, "plain"), args.plain_steps, 1)
+    print(f"# plain fwd+bwd: {ms_plain:.3f} ms/step", file=sys.stderr, flush=True)
+
+    f_ms, b_ms, out = kernel_times(torch, qpax_amd, ts, reps=max(3, min(args.steps, 10)))
+    it_f = out[5].cpu().numpy().astype(np.float64)
+    it_r = out[11].cpu().numpy().astype(np.float64)
+    fl_fwd = sum(flops_per_qp(N, ME, MI, a, b, bwd=False)[0] for a, b in zip(it_f, it_r))
+    achieved = fl_fwd / (f_ms * 1e-3) / 1e12
+
+    x_gpu = fwd_bwd(qpax_amd, ts, "fused").detach().cpu().numpy()
+    g_gpu = [t.grad.detach().cpu().numpy() for t in ts]
+
+    value = BATCH / (ms_fused * 1e-3)
+    res = {
+        "metric": "fwd+bwd QPs/s @n=50,B=4096: fused vs plain GPU; x/grad err vs FP64",
+        "value": round(value, 1),
+        "unit": "QPs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_fused, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "solve_qp_primal fwd+bwd, loss ||x-1||^2, 4096 random feasible QPs n=50 m_eq=10 m_ineq=100, "
+                        "float32 inputs/outputs, float64 arithmetic, solver_tol=1e-5, target_kappa=1e-3, seed 3",
+            "batch": BATCH, "n": N, "m_eq": ME, "m_ineq": MI, "io_dtype": "f32",
+        },
+        "plain_qps": round(BATCH / (ms_plain * 1e-3), 1),
+        "plain_ms_per_step": round(ms_plain, 3),
+        "fused_over_plain": round(ms_plain / ms_fused, 2),
+        "iters_fwd_mean": float(it_f.mean()),
+        "iters_relax_mean": float(it_r.mean()),
+        "kernel_ms": {"fused_solve_relax": round(f_ms, 4), "fused_backward": round(b_ms, 4)},
+        "roofline": {
+            "kernel": "qpx::k_fused_solve (solve + relax, float64 VALU)",
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_F64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_F64_TFLOPS, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        cpu_qps, xo, go, info = cpu_baseline(data, args.cpu_sample, None)
+        S = args.cpu_sample
+        ok = (info[0] == 1) & (info[2] == 1)
+
+        def rel(a, b):
+            a = a.reshape(a.shape[0], -1)
+            b = b.reshape(b.shape[0], -1)
+            return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-12)
+
+        ex = rel(x_gpu[:S], xo)
+        eg = [rel(g[:S], r) for g, r in zip(g_gpu, go)]
+        res["x_rel_err_vs_f64_oracle"] = {"max_converged": float(ex[ok].max()), "median": float(np.median(ex)),
+                                          "oracle_converged_frac": float(ok.mean()), "sample": S}
+        res["grad_rel_err_vs_f64_oracle"] = {k: float(e[ok].max()) for k, e in zip("Q q A b G h".split(), eg)}
+        res["cpu_baseline"] = {"value": round(cpu_qps, 2), "unit": "QPs/s", "cores": 1, "kind": "port",
+                               "sample": f"first {S} of the 4096 QPs, fwd (solve+relax) + bwd, float64 numpy oracle "
+                                         f"(QR solves), 1 thread"}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,26 +15,16 @@
 #include <type_traits>
 
 #include "../../include/qpx.h"
+#include "qpx_args.hpp"
 #include "qpx_core.hpp"
 
 namespace qpx {
 
-template <typename TS>
-struct Args {
-  int64_t B;
-  int n, me, mi;
-  const TS *Q, *q, *A, *b, *G, *h;
-  TS *x, *s, *z, *y;
-  int *conv, *iters;
-  TS *xr, *sr, *zr, *yr;
-  int *rconv, *riters;
-  const TS *xr_in, *sr_in, *zr_in, *yr_in, *g;
-  TS *dQ, *dq, *dA, *db, *dG, *dh;
-  double* ws;
-  int64_t stride;  // float64 words per problem
-  unsigned* counter;
-  double tol, kappa;
-};
+
+
+}  // namespace qpx
+#include "qpx_wave_launch.hpp"
+namespace qpx {
 
 template <typename TS>
 __device__ __forceinline__ View view_of(const Args<TS>& a, int64_t p) {
@@ -306,11 +296,59 @@ int* pinned_counter() {
   return p;
 }
 
+// The wavefront-per-QP kernels hold the augmented KKT row of a lane in registers, so they take
+// problems with n + m_eq <= 64 and m_ineq <= 256 whose LDS image fits one workgroup.
+int max_lds_per_block() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0;
+    int val = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) val = 65536;
+    v = val;
+  }
+  return v;
+}
+
+template <typename TS>
+bool wave_fits(int n, int me, int mi) {
+  if (n + me > 64 || mi > 256) return false;
+  return qpx::wave::lds_layout(n, me, mi, (int)sizeof(TS)).bytes <= max_lds_per_block();
+}
+
+#define QPX_WAVE_CASES(TS, X)                  \
+  if (m <= 32 && mi <= 64) return X(TS, 32, 1);  \
+  if (m <= 32 && mi <= 128) return X(TS, 32, 2); \
+  if (mi <= 64) return X(TS, 64, 1);             \
+  if (mi <= 128) return X(TS, 64, 2);            \
+  return X(TS, 64, 4);
+
+template <typename TS>
+hipError_t wave_solve(const qpx::Args<TS>& a, int relax, hipStream_t st) {
+  const int m = a.n + a.me, mi = a.mi;
+  const int lds = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
+#define QPX_X(T, N, M) qpx::wave::launch_wave_solve<T, N, M>(a, (unsigned)a.B, lds, st, relax)
+  QPX_WAVE_CASES(TS, QPX_X)
+#undef QPX_X
+}
+
+template <typename TS>
+hipError_t wave_bwd(const qpx::Args<TS>& a, hipStream_t st) {
+  const int m = a.n + a.me, mi = a.mi;
+  const int lds = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
+#define QPX_X(T, N, M) qpx::wave::launch_wave_bwd<T, N, M>(a, (unsigned)a.B, lds, st)
+  QPX_WAVE_CASES(TS, QPX_X)
+#undef QPX_X
+}
+
 template <int NT, typename TS>
 qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
+    if (wave_fits<TS>(a.n, a.me, a.mi)) {
+      QPX_HIP(wave_solve<TS>(a, relax ? 1 : 0, st));
+      return QPX_OK;
+    }
     hipLaunchKernelGGL((qpx::k_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
     QPX_HIP(hipGetLastError());
     return QPX_OK;
@@ -347,6 +385,10 @@ qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
+    if (wave_fits<TS>(a.n, a.me, a.mi)) {
+      QPX_HIP(wave_bwd<TS>(a, st));
+      return QPX_OK;
+    }
     hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, 0, st, a);
   } else {
     hipLaunchKernelGGL((qpx::k_bwd_load<NT, TS>), grid, block, 0, st, a);

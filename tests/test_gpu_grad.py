@@ -47,10 +47,10 @@ def test_grads_match_oracle(name, gen, dt, gtol):
         assert e.max() < gtol
 
 
-def test_grads_plain_equals_fused():
-    data = [np.asarray(v, np.float32) for v in P.random_feasible(16, 50, 10, 100, seed=7)]
-    xa, ga = _grads(data, np.float32, "fused")
-    xb, gb = _grads(data, np.float32, "plain")
-    np.testing.assert_array_equal(xa, xb)
+def test_grads_plain_agrees_with_fused():
+    data = [np.asarray(v, np.float64) for v in P.random_feasible(16, 50, 10, 100, seed=7)]
+    xa, ga = _grads(data, np.float64, "fused")
+    xb, gb = _grads(data, np.float64, "plain")
+    assert rel_err(xa, xb).max() < 1e-9
     for a, b in zip(ga, gb):
-        np.testing.assert_array_equal(a, b)
+        assert rel_err(a, b).max() < 1e-6

@@ -49,9 +49,13 @@ def test_solve_matches_oracle(name, gen, dt, tol, xtol):
         assert rel_err(a[ok], b[ok]).max() < max(xtol, 1e-6) * 10
 
 
-@pytest.mark.parametrize("name,gen,dt,tol,xtol", CASES[:4], ids=[c[0] for c in CASES[:4]])
-def test_plain_equals_fused(name, gen, dt, tol, xtol):
-    _, a = _run(name, gen, dt, tol, "fused")
-    _, b = _run(name, gen, dt, tol, "plain")
-    for u, v in zip(a, b):
-        np.testing.assert_array_equal(u, v)
+@pytest.mark.parametrize("name,gen,dt,tol,xtol", CASES, ids=[c[0] for c in CASES])
+def test_plain_matches_oracle(name, gen, dt, tol, xtol):
+    """The plain per-stage path (a different elimination order than the fused one) meets the same
+    bar against the oracle."""
+    data, (x, s, z, y, conv, iters) = _run(name, gen, dt, tol, "plain")
+    xo, so, zo, yo, co, io = O.solve_qp(*[d.astype(np.float64) for d in data], solver_tol=tol)
+    ok = co == 1
+    np.testing.assert_array_equal(conv, co)
+    assert np.abs(iters - io).max() <= 1
+    assert rel_err(x[ok], xo[ok]).max() < xtol
