@@ -47,9 +47,7 @@ __device__ __forceinline__ double wsum(double v) {
 
 template <typename TS, int NMAX, int MIC>
 struct St {
-  // constant problem data
-  double kq[NMAX];
-  double kqd;     // diagonal of kq in this lane's row
+  // constant problem data (the augmented matrix itself lives in the workspace, see Ctx::KQ)
   double qa;      // q (rows < n), -b (rows n..m-1)
   double h[MIC];
   bool hinf[MIC];
@@ -64,6 +62,9 @@ struct St {
 template <typename TS, int NMAX, int MIC>
 struct Ctx {
   int n, me, mi, m, l;
+  // K = [[Q_sym, A^T], [A, 0]] (m x m, float64, symmetric) in this problem's global workspace;
+  // lane l reads column entries KQ[j*m + l] = K[j][l] = K[l][j], coalesced across the lanes.
+  double* KQ;
   const TS* Gs;
   const TS* As;
   double* Lp;
@@ -72,6 +73,52 @@ struct Ctx {
 };
 
 #define WSYNC() __syncthreads()
+
+// sum_k G[k][col] * v[k] over k < mi (v broadcast from LDS); four rows per trip.
+template <typename TS>
+__device__ __forceinline__ double gcol_dot(const TS* __restrict__ Gs, int n, int mi, int col,
+                                           const double* __restrict__ v) {
+  double a0 = 0.0, a1 = 0.0;
+  int k = 0;
+  for (; k + 4 <= mi; k += 4) {
+    const double2 v01 = *reinterpret_cast<const double2*>(v + k);
+    const double2 v23 = *reinterpret_cast<const double2*>(v + k + 2);
+    const double g0 = (double)Gs[k * n + col], g1 = (double)Gs[(k + 1) * n + col];
+    const double g2 = (double)Gs[(k + 2) * n + col], g3 = (double)Gs[(k + 3) * n + col];
+    a0 += g0 * v01.x;
+    a1 += g1 * v01.y;
+    a0 += g2 * v23.x;
+    a1 += g3 * v23.y;
+  }
+  for (; k < mi; ++k) a0 += (double)Gs[k * n + col] * v[k];
+  return a0 + a1;
+}
+
+// sum_j G[row][j] * v[j] over j < n (v broadcast from LDS); four columns per trip.
+template <typename TS>
+__device__ __forceinline__ double grow_dot(const TS* __restrict__ Gs, int n, int row, const double* __restrict__ v) {
+  const TS* g = Gs + row * n;
+  double a0 = 0.0, a1 = 0.0;
+  int j = 0;
+  for (; j + 4 <= n; j += 4) {
+    const double2 v01 = *reinterpret_cast<const double2*>(v + j);
+    const double2 v23 = *reinterpret_cast<const double2*>(v + j + 2);
+    a0 += (double)g[j] * v01.x;
+    a1 += (double)g[j + 1] * v01.y;
+    a0 += (double)g[j + 2] * v23.x;
+    a1 += (double)g[j + 3] * v23.y;
+  }
+  for (; j < n; ++j) a0 += (double)g[j] * v[j];
+  return a0 + a1;
+}
+
+// 1/d to full double precision: hardware reciprocal plus two Newton steps.
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
 
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, const TS* Q,
@@ -94,18 +141,19 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
     st.hinf[cc] = isinf(hk);
     st.h[cc] = st.hinf[cc] ? 0.0 : hk;
   }
-#pragma unroll
-  for (int j = 0; j < NMAX; ++j) {
-    double v = 0.0;
-    if (l < n) {
-      if (j < n) v = 0.5 * ((double)Q[l * n + j] + (double)Q[j * n + l]);
-      else if (j < m) v = (double)A[(j - n) * n + l];
-    } else if (l < m) {
-      if (j < n) v = (double)A[(l - n) * n + j];
+  if (l < m) {
+    for (int j = 0; j < m; ++j) {
+      double v = 0.0;
+      if (l < n) {
+        if (j < n) v = 0.5 * ((double)Q[l * n + j] + (double)Q[j * n + l]);
+        else v = (double)A[(j - n) * n + l];
+      } else if (j < n) {
+        v = (double)A[(l - n) * n + j];
+      }
+      c.KQ[j * m + l] = v;
     }
-    st.kq[j] = v;
   }
-  st.kqd = l < n ? (double)Q[l * n + l] : 0.0;
+  __threadfence_block();
   st.qa = l < n ? (double)q[l] : (l < m ? -(double)b[l - n] : 0.0);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) { st.s[cc] = 1.0; st.z[cc] = 1.0; }
@@ -125,27 +173,26 @@ __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, 
   }
   WSYNC();
   double acc = st.qa;
-#pragma unroll
-  for (int j = 0; j < NMAX; j += 2) {
-    if (j < m) {
+  if (l < m) {
+    const double* kcol = c.KQ + l;
+    double a1 = 0.0;
+    int j = 0;
+    for (; j + 2 <= m; j += 2) {
       const double2 x2 = *reinterpret_cast<const double2*>(c.col + j);
-      acc += st.kq[j] * x2.x;
-      acc += st.kq[j + 1] * x2.y;
+      acc += kcol[j * m] * x2.x;
+      a1 += kcol[(j + 1) * m] * x2.y;
     }
+    if (j < m) acc += kcol[j * m] * c.col[j];
+    acc += a1;
   }
-  if (l < n) {
-    const TS* gcol = c.Gs + l;
-    for (int k = 0; k < mi; ++k) acc += (double)gcol[k * n] * c.vb[k];
-  }
+  if (l < n) acc += gcol_dot(c.Gs, n, mi, l, c.vb);
   st.ra = l < m ? acc : 0.0;
   double nrm = fabs(st.ra);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if (k < mi) {
-      const TS* grow = c.Gs + k * n;
-      double g = 0.0;
-      for (int j = 0; j < n; ++j) g += (double)grow[j] * c.col[j];
+      const double g = grow_dot(c.Gs, n, k, c.col);
       st.r3[cc] = g + st.s[cc] - st.h[cc];
       st.r2[cc] = st.s[cc] * st.z[cc] - kappa;
       nrm = nmax(nrm, nmax(fabs(st.r2[cc]), fabs(st.r3[cc])));
@@ -199,8 +246,8 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
   // Move the tiles into row-per-lane layout through LDS (staging area = the factor's region).
   // Tile element (i, j) sits in lane j + 16*(i%4), register i/4.
 #pragma unroll
-  for (int j = 0; j < NMAX; ++j) st.kr[j] = st.kq[j];
-  double diag = st.kqd;
+  for (int j = 0; j < NMAX; ++j) st.kr[j] = (j < m && l < m) ? c.KQ[j * m + l] : 0.0;
+  double diag = l < m ? c.KQ[l * m + l] : 0.0;
   {
     int t = 0;
 #pragma unroll
@@ -242,22 +289,24 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
       const bool hp = j < n;
       const bool ok = hp ? (dj > kPivotRel * fabs(d0j)) : (dj < -kPivotRel * fabs(d0j));
       const double d = ok ? dj : (hp ? kHugePivot : -kHugePivot);
-      const double inv = 1.0 / d;
+      const double inv = recip(d);
       if (l == j) st.rdpiv = inv;
       double* cb = c.col + (j & 1) * 64;
       const double cv = st.kr[j];
       cb[l] = cv;
       WSYNC();
-      const double lf = cv * inv;
+      // rows above the pivot (and padding rows, which are zero) take no update
+      const double lf = (l > j) ? cv * inv : 0.0;
       diag -= lf * cv;
+      // columns k >= m hold zeros in every row, so the update runs to NMAX unguarded
 #pragma unroll
-      for (int k = j + 1; k < NMAX; k += 1) {
-        if (k < m) st.kr[k] -= lf * cb[k];
+      for (int k = (j + 1) & ~1; k < NMAX; k += 2) {
+        const double2 c2 = *reinterpret_cast<const double2*>(cb + k);
+        if (k > j) st.kr[k] -= lf * c2.x;
+        st.kr[k + 1] -= lf * c2.y;
       }
-      if (l > j && l < m) {
-        st.kr[j] = lf;
-        c.Lp[l * (l + 1) / 2 + j] = lf;
-      }
+      st.kr[j] = lf;
+      if (l > j && l < m) c.Lp[l * (l + 1) / 2 + j] = lf;
     }
   }
   WSYNC();
@@ -277,24 +326,20 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
     if (k < mi) c.vb[k] = st.D[cc] * (sg * a3[cc] - (sg * a2[cc]) / st.z[cc]);
   }
   WSYNC();
-  if (l < n) {
-    const TS* gcol = c.Gs + l;
-    for (int k = 0; k < mi; ++k) pa += (double)gcol[k * n] * c.vb[k];
-  }
+  if (l < n) pa += gcol_dot(c.Gs, n, mi, l, c.vb);
   if (l >= m) pa = 0.0;
-  // forward: unit-lower L y = p
+  // forward: unit-lower L y = p; kr[j] is zero in rows <= j
 #pragma unroll
   for (int j = 0; j < NMAX; ++j) {
-    if (j < m) {
-      const double yj = rl(pa, j);
-      if (l > j) pa -= st.kr[j] * yj;
-    }
+    if (j < m) pa -= st.kr[j] * rl(pa, j);
   }
   pa *= st.rdpiv;
-  // backward: L^T x = y, rows of L from the packed copy in LDS
+  // backward: L^T x = y, rows of L from the packed copy in LDS.  The read for lanes >= j runs
+  // past row j into the next rows / the broadcast buffers that follow the factor (in bounds of
+  // this wavefront's LDS image) and is masked off.
   for (int j = m - 1; j > 0; --j) {
-    const double xj = rl(pa, j);
-    if (l < j) pa -= c.Lp[j * (j + 1) / 2 + l] * xj;
+    const double lv = c.Lp[j * (j + 1) / 2 + l];
+    pa -= (l < j ? lv : 0.0) * rl(pa, j);
   }
   st.da = l < m ? pa : 0.0;
   c.col[l] = st.da;
@@ -303,9 +348,7 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if (k < mi) {
-      const TS* grow = c.Gs + k * n;
-      double g = 0.0;
-      for (int j = 0; j < n; ++j) g += (double)grow[j] * c.col[j];
+      const double g = grow_dot(c.Gs, n, k, c.col);
       const double v3 = sg * a3[cc], v2 = sg * a2[cc];
       const double dsk = v3 - g;
       ods[cc] = dsk;
@@ -413,7 +456,7 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
 }
 
 template <typename TS, int NMAX, int MIC>
-__device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi) {
+__device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, double* ws) {
   extern __shared__ __align__(16) unsigned char qpx_smem[];
   const Lds w = lds_layout(n, me, mi, (int)sizeof(TS));
   Ctx<TS, NMAX, MIC> c;
@@ -423,6 +466,7 @@ __device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi) {
   c.Lp = reinterpret_cast<double*>(qpx_smem + w.off_L);
   c.col = reinterpret_cast<double*>(qpx_smem + w.off_c);
   c.vb = reinterpret_cast<double*>(qpx_smem + w.off_v);
+  c.KQ = ws;
   return c;
 }
 
@@ -449,7 +493,7 @@ template <typename TS, int NMAX, int MIC>
 __global__ __launch_bounds__(64) void k_wave_solve(Args<TS> a, int with_relax) {
   const int64_t p = blockIdx.x;
   const int n = a.n, me = a.me, mi = a.mi;
-  const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi);
+  const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + p * a.stride);
   St<TS, NMAX, MIC> st;
   load(c, st, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me, a.G + p * mi * n, a.h + p * mi);
   init(c, st);
@@ -479,7 +523,7 @@ template <typename TS, int NMAX, int MIC>
 __global__ __launch_bounds__(64) void k_wave_bwd(Args<TS> a) {
   const int64_t p = blockIdx.x;
   const int n = a.n, me = a.me, mi = a.mi, m = n + me;
-  const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi);
+  const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + p * a.stride);
   const int l = c.l;
   St<TS, NMAX, MIC> st;
   load(c, st, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me, a.G + p * mi * n, a.h + p * mi);
