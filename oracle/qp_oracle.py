@@ -274,23 +274,31 @@ def relax_qp(Q, q, A, b, G, h, x, s, z, y, solver_tol=1e-5, target_kappa=1e-5):
     return x, s, z, y, conv, iters
 
 
-def optnet_derivatives(dz, dlam, dnu, z, lam, nu):
-    """qpax/diff_qp.py:8-17 -- parameter cotangents from the adjoint KKT solution."""
+def optnet_derivatives(dz, dlam, dnu, z, lam, nu, exact_dG=False):
+    """qpax/diff_qp.py:8-17 -- parameter cotangents from the adjoint KKT solution.
+
+    The reference scales both terms of dl_dG by diag(lam) (diff_qp.py:11), which makes its second
+    term lam^2 dz^T; the exact vector-Jacobian product of the relaxed solution map is
+    diag(lam) dlam z^T + lam dz^T.  exact_dG=True returns the latter (checked against central
+    finite differences); the default restates the reference."""
     outer = lambda u, v: u[..., :, None] * v[..., None, :]  # noqa: E731
     dl_dQ = 0.5 * (outer(dz, z) + outer(z, dz))
     dl_dA = outer(dnu, z) + outer(nu, dz)
-    dl_dG = lam[..., :, None] * (outer(dlam, z) + outer(lam, dz))
+    if exact_dG:
+        dl_dG = lam[..., :, None] * outer(dlam, z) + outer(lam, dz)
+    else:
+        dl_dG = lam[..., :, None] * (outer(dlam, z) + outer(lam, dz))
     return dl_dQ, dz, dl_dA, -dnu, dl_dG, -lam * dlam
 
 
-def diff_qp(Q, q, A, b, G, h, z, s, lam, nu, dl_dz):
+def diff_qp(Q, q, A, b, G, h, z, s, lam, nu, dl_dz, exact_dG=False):
     """qpax/diff_qp.py:20-46 -- factor at the relaxed point, solve with rhs (-dl/dx, 0, 0, 0)."""
     Q, q, A, b, G, h = _as_batch(Q, q, A, b, G, h)
     P_inv_vec, L_H, L_F = factorize_kkt(Q, G, A, s, lam)
     zs = np.zeros_like(h)
     dz, ds, dlam_tilde, dnu = solve_kkt_rhs(Q, G, A, s, lam, P_inv_vec, L_H, L_F, -dl_dz, zs, zs, np.zeros_like(b))
     dlam = dlam_tilde / lam
-    return optnet_derivatives(dz, dlam, dnu, z, lam, nu)
+    return optnet_derivatives(dz, dlam, dnu, z, lam, nu, exact_dG)
 
 
 def solve_qp_primal_fwd(Q, q, A, b, G, h, solver_tol=1e-5, target_kappa=1e-3):
@@ -302,10 +310,10 @@ def solve_qp_primal_fwd(Q, q, A, b, G, h, solver_tol=1e-5, target_kappa=1e-3):
     return x, (xr, sr, zr, yr), (c1, i1, c2, i2)
 
 
-def solve_qp_primal_bwd(Q, q, A, b, G, h, relaxed, dl_dx):
+def solve_qp_primal_bwd(Q, q, A, b, G, h, relaxed, dl_dx, exact_dG=False):
     """qpax/diff_qp.py:74-79 -- the six cotangents (Q, q, A, b, G, h)."""
     xr, sr, zr, yr = relaxed
-    return diff_qp(Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx)
+    return diff_qp(Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, exact_dG)
 
 
 # ----------------------------------------------------------------------------------------------

@@ -27,6 +27,9 @@ FLAGS = [
     "-fPIC",
     "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: avoids AGPR<->VGPR copies around every MFMA when the
+    # accumulators are loop-carried next to a large register-resident working set
+    "-mllvm", "-amdgpu-mfma-vgpr-form=true",
     "-Wno-unused-command-line-argument",
 ]
 

@@ -259,7 +259,13 @@ qpx_status fail(qpx_status st, const char* fmt, ...) {
 
 constexpr size_t kCtlBytes = 4096;  // head of the workspace: counters
 
-int64_t stride_words(int n, int me, int mi) { return qpx::view_words(n, me, mi); }
+// Per-problem float64 words: the generic kernels' view, or the wavefront kernels' m x m reduced
+// KKT matrix (m = n padded to a multiple of 4, plus m_eq), whichever is larger.
+int64_t stride_words(int n, int me, int mi) {
+  const int64_t m = qpx::wave::padded_n(n) + me;
+  const int64_t w = qpx::view_words(n, me, mi);
+  return w > qpx::round_up(m * m, 64) ? w : qpx::round_up(m * m, 64);
+}
 
 size_t ws_bytes(int64_t B, int n, int me, int mi) {
   return kCtlBytes + (size_t)B * (size_t)stride_words(n, me, mi) * sizeof(double);
@@ -311,7 +317,7 @@ int max_lds_per_block() {
 
 template <typename TS>
 bool wave_fits(int n, int me, int mi) {
-  if (n + me > 64 || mi > 256) return false;
+  if (qpx::wave::padded_n(n) + me > 64 || mi > 256) return false;
   return qpx::wave::lds_layout(n, me, mi, (int)sizeof(TS)).bytes <= max_lds_per_block();
 }
 
@@ -324,7 +330,7 @@ bool wave_fits(int n, int me, int mi) {
 
 template <typename TS>
 hipError_t wave_solve(const qpx::Args<TS>& a, int relax, hipStream_t st) {
-  const int m = a.n + a.me, mi = a.mi;
+  const int m = qpx::wave::padded_n(a.n) + a.me, mi = a.mi;
   const int lds = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
 #define QPX_X(T, N, M) qpx::wave::launch_wave_solve<T, N, M>(a, (unsigned)a.B, lds, st, relax)
   QPX_WAVE_CASES(TS, QPX_X)
@@ -333,7 +339,7 @@ hipError_t wave_solve(const qpx::Args<TS>& a, int relax, hipStream_t st) {
 
 template <typename TS>
 hipError_t wave_bwd(const qpx::Args<TS>& a, hipStream_t st) {
-  const int m = a.n + a.me, mi = a.mi;
+  const int m = qpx::wave::padded_n(a.n) + a.me, mi = a.mi;
   const int lds = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
 #define QPX_X(T, N, M) qpx::wave::launch_wave_bwd<T, N, M>(a, (unsigned)a.B, lds, st)
   QPX_WAVE_CASES(TS, QPX_X)

@@ -1,18 +1,22 @@
 // Fused solver for small dense QPs: one wavefront owns one QP and keeps it on chip.
 //
-// Layout (lane l = 0..63 of the wavefront, m = n + m_eq <= NMAX <= 64):
-//   * registers: row l of the augmented KKT matrix K = [[Q, A^T], [A, 0]] (kq, constant) and of
-//     the working matrix / unit-lower factor (kr); the augmented vectors [x; y], [r1; r4],
-//     [dx; dy] one element per lane; the m_ineq-vectors (s, z, D, residuals, directions) in
-//     MIC chunks of 64 rows, row 64c + l in lane l.
-//   * LDS: G and A (input precision), the packed lower factor (for the transposed solve), two
-//     64-entry broadcast buffers and one m_ineq-entry broadcast buffer.
-// The reduced KKT system of qpax's cvxopt reduction (qpax/pdip.py:38-63),
-//     [[H, A^T], [A, 0]] [dx; dy] = [p1; v4],   H = Q + G^T diag(z/s) G,
-// is factored as one quasi-definite LDL^T (pivots 0..n-1 positive: the Cholesky of H; pivots
-// n..m-1 negative: minus the Schur complement F = A H^{-1} A^T), which is the same elimination
-// as the reference's H-then-F factorisation in a single pass.  G^T D G runs on the float64
-// matrix cores (v_mfma_f64_16x16x4); everything else on the vector ALUs.
+// Rows of the reduced KKT system (lane index l = 0..63):
+//   0 .. n-1      x            (the QP's variables)
+//   n .. np-1     padding      (np = n rounded up to a multiple of 4; identity rows)
+//   np .. m-1     y            (m = np + m_eq <= NMAX <= 64; the equality multipliers)
+// The cvxopt reduction of qpax (qpax/pdip.py:38-63) solves
+//     K [dx; dy] = [p1; v4],   K = [[H, A^T], [A, 0]],   H = Q + G^T diag(z/s) G.
+// K is factored as one quasi-definite LDL^T: pivots 0..np-1 are positive (the Cholesky of H),
+// pivots np..m-1 negative (minus the Schur complement F = A H^{-1} A^T) -- the reference's
+// H-then-F factorisation done as a single elimination.
+//
+// Registers hold K in the float64 matrix-core tile layout (16x16 tiles of the lower triangle,
+// element (16I + 4r + l/16, 16J + l%16) of tile (I, J) in register r of lane l), the augmented
+// vectors [x; y] one element per lane, and the m_ineq-vectors in MIC chunks of 64 rows (row
+// 64c + l in lane l).  G^T D G accumulates into the tiles with v_mfma_f64_16x16x4; the
+// elimination runs in panels of 4 pivots whose rank-4 trailing update is one such MFMA per tile.
+// LDS holds G (input precision), the packed unit-lower factor (row l at l(l+1)/2) for the two
+// triangular sweeps, and a panel/broadcast buffer.
 #pragma once
 #include "qpx_args.hpp"
 #include "qpx_core.hpp"
@@ -29,87 +33,69 @@ __device__ __forceinline__ double rl(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double wmax(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o, 64));
-  return v;
+// Wavefront reductions: butterfly inside each row of 16 lanes with DPP (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then the four row results through readlane.
+// Every lane receives the result.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double wmin(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = nmin(v, __shfl_xor(v, o, 64));
-  return v;
+template <class Op>
+__device__ __forceinline__ double wreduce(double v, Op op) {
+  v = op(v, dpp<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = op(v, dpp<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = op(v, dpp<0x141>(v));  // row_half_mirror
+  v = op(v, dpp<0x140>(v));  // row_mirror
+  return op(op(rl(v, 0), rl(v, 16)), op(rl(v, 32), rl(v, 48)));
 }
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ double wmax(double v) { return wreduce(v, OpMax()); }
+__device__ __forceinline__ double wmin(double v) { return wreduce(v, OpMin()); }
+__device__ __forceinline__ double wsum(double v) { return wreduce(v, OpSum()); }
 
-template <typename TS, int NMAX, int MIC>
-struct St {
-  // constant problem data (the augmented matrix itself lives in the workspace, see Ctx::KQ)
-  double qa;      // q (rows < n), -b (rows n..m-1)
-  double h[MIC];
-  bool hinf[MIC];
-  // factor
-  double kr[NMAX];
-  double rdpiv;
-  // iterate and work vectors
-  double xa, ra, da;
-  double s[MIC], z[MIC], D[MIC], r2[MIC], r3[MIC], dsa[MIC], dza[MIC], ds[MIC], dz[MIC];
-};
-
-template <typename TS, int NMAX, int MIC>
-struct Ctx {
-  int n, me, mi, m, l;
-  // K = [[Q_sym, A^T], [A, 0]] (m x m, float64, symmetric) in this problem's global workspace;
-  // lane l reads column entries KQ[j*m + l] = K[j][l] = K[l][j], coalesced across the lanes.
-  double* KQ;
-  const TS* Gs;
-  const TS* As;
-  double* Lp;
-  double* col;
-  double* vb;
-};
-
-#define WSYNC() __syncthreads()
+#define WSYNC() __builtin_amdgcn_wave_barrier()
 
 // sum_k G[k][col] * v[k] over k < mi (v broadcast from LDS); four rows per trip.
 template <typename TS>
 __device__ __forceinline__ double gcol_dot(const TS* __restrict__ Gs, int n, int mi, int col,
                                            const double* __restrict__ v) {
-  double a0 = 0.0, a1 = 0.0;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
   int k = 0;
-  for (; k + 4 <= mi; k += 4) {
-    const double2 v01 = *reinterpret_cast<const double2*>(v + k);
-    const double2 v23 = *reinterpret_cast<const double2*>(v + k + 2);
-    const double g0 = (double)Gs[k * n + col], g1 = (double)Gs[(k + 1) * n + col];
-    const double g2 = (double)Gs[(k + 2) * n + col], g3 = (double)Gs[(k + 3) * n + col];
-    a0 += g0 * v01.x;
-    a1 += g1 * v01.y;
-    a0 += g2 * v23.x;
-    a1 += g3 * v23.y;
+  for (; k + 8 <= mi; k += 8) {
+    double g[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) g[u] = (double)Gs[(k + u) * n + col];
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      const double2 vv = *reinterpret_cast<const double2*>(v + k + u);
+      a[(u >> 1) & 3] += g[u] * vv.x;
+      a[(u >> 1) & 3] += g[u + 1] * vv.y;
+    }
   }
-  for (; k < mi; ++k) a0 += (double)Gs[k * n + col] * v[k];
-  return a0 + a1;
+  for (; k < mi; ++k) a[0] += (double)Gs[k * n + col] * v[k];
+  return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
 // sum_j G[row][j] * v[j] over j < n (v broadcast from LDS); four columns per trip.
 template <typename TS>
 __device__ __forceinline__ double grow_dot(const TS* __restrict__ Gs, int n, int row, const double* __restrict__ v) {
   const TS* g = Gs + row * n;
-  double a0 = 0.0, a1 = 0.0;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
   int j = 0;
-  for (; j + 4 <= n; j += 4) {
-    const double2 v01 = *reinterpret_cast<const double2*>(v + j);
-    const double2 v23 = *reinterpret_cast<const double2*>(v + j + 2);
-    a0 += (double)g[j] * v01.x;
-    a1 += (double)g[j + 1] * v01.y;
-    a0 += (double)g[j + 2] * v23.x;
-    a1 += (double)g[j + 3] * v23.y;
+  for (; j + 8 <= n; j += 8) {
+    double gg[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) gg[u] = (double)g[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      const double2 vv = *reinterpret_cast<const double2*>(v + j + u);
+      a[(u >> 1) & 3] += gg[u] * vv.x;
+      a[(u >> 1) & 3] += gg[u + 1] * vv.y;
+    }
   }
-  for (; j < n; ++j) a0 += (double)g[j] * v[j];
-  return a0 + a1;
+  for (; j < n; ++j) a[0] += (double)g[j] * v[j];
+  return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
 // 1/d to full double precision: hardware reciprocal plus two Newton steps.
@@ -120,48 +106,73 @@ __device__ __forceinline__ double recip(double d) {
   return r;
 }
 
+
+template <typename TS, int NMAX, int MIC>
+struct St {
+  double qa;      // q (rows < n), -b (rows np..m-1), 0 otherwise
+  double h[MIC];
+  bool hinf[MIC];
+  double rdpiv;   // 1 / pivot of this lane's row
+  double xa, ra, da;
+  double s[MIC], z[MIC], D[MIC], r2[MIC], r3[MIC], dsa[MIC], dza[MIC], ds[MIC], dz[MIC];
+};
+
+template <typename TS, int NMAX, int MIC>
+struct Ctx {
+  int n, np, me, mi, m, l;
+  // K's constant part [[Q_sym, A^T], [A, 0]] (m x m, float64, symmetric, identity on the padding
+  // rows) in this problem's global workspace; entry (i, j) at KQ[j*m + i].
+  double* KQ;
+  const TS* Gs;   // G, row-major mi x n
+  double* Lp;     // packed unit-lower factor
+  double* pb;     // panel buffer (64 x 8 doubles); outside the factorisation: col + vb
+  double* col;    // 128 doubles (aliases pb)
+  double* vb;     // m_ineq doubles (aliases pb + 128)
+};
+
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, const TS* Q,
                                      const TS* q, const TS* A, const TS* b, const TS* G, const TS* h) {
-  const int n = c.n, me = c.me, mi = c.mi, m = c.m, l = c.l;
+  const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l;
   TS* Gs = const_cast<TS*>(c.Gs);
-  TS* As = const_cast<TS*>(c.As);
   for (int k = 0; k < mi; ++k) {
     const bool inf = isinf((double)h[k]);
     for (int j = l; j < n; j += 64) {
       const TS g = G[k * n + j];
-      Gs[k * n + j] = inf ? (TS)0 * g : g;
+      Gs[k * n + j] = inf ? (TS)0 * g : g;  // diag(1 * ~mask) @ G (pdip.py:176)
     }
   }
-  for (int e = l; e < me * n; e += 64) As[e] = A[e];
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     const double hk = k < mi ? (double)h[k] : 0.0;
     st.hinf[cc] = isinf(hk);
     st.h[cc] = st.hinf[cc] ? 0.0 : hk;
+    st.s[cc] = 1.0;
+    st.z[cc] = 1.0;
   }
   if (l < m) {
     for (int j = 0; j < m; ++j) {
       double v = 0.0;
       if (l < n) {
         if (j < n) v = 0.5 * ((double)Q[l * n + j] + (double)Q[j * n + l]);
-        else v = (double)A[(j - n) * n + l];
+        else if (j >= np) v = (double)A[(j - np) * n + l];
+      } else if (l < np) {
+        v = (j == l) ? 1.0 : 0.0;
       } else if (j < n) {
-        v = (double)A[(l - n) * n + j];
+        v = (double)A[(l - np) * n + j];
       }
       c.KQ[j * m + l] = v;
     }
   }
   __threadfence_block();
-  st.qa = l < n ? (double)q[l] : (l < m ? -(double)b[l - n] : 0.0);
-#pragma unroll
-  for (int cc = 0; cc < MIC; ++cc) { st.s[cc] = 1.0; st.z[cc] = 1.0; }
+  st.qa = l < n ? (double)q[l] : ((l >= np && l < m) ? -(double)b[l - np] : 0.0);
   st.xa = 0.0;
   WSYNC();
 }
 
-// Residuals of the (relaxed) KKT system; returns the infinity norm (NaN-propagating).
+// Residuals of the (relaxed) KKT system (qpax/pdip.py:93-100); returns the infinity norm
+// (NaN-propagating).
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double kappa) {
   const int n = c.n, mi = c.mi, m = c.m, l = c.l;
@@ -205,116 +216,171 @@ __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, 
   return wmax(nrm);
 }
 
-// D = z/s (or 1), K = kq + G^T D G on the top-left block, LDL^T of K in place.
+__host__ __device__ constexpr int tidx(int I, int J) { return I * (I + 1) / 2 + J; }
+
+// Factorisation (qpax/pdip.py:38-46): D = z/s (or 1 for the initialisation), K = KQ + G^T D G,
+// K = L D L^T with L unit lower (rows packed into c.Lp) and 1/D in st.rdpiv.  A pivot that is not
+// beyond kPivotRel times its reference diagonal -- the diagonal entry before elimination for the
+// H block, the diagonal entry once all H pivots are done for the Schur block -- or is NaN is
+// replaced by kHugePivot with the block's sign, which zeroes that component of later solves.
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool unit_d) {
   constexpr int NTI = NMAX / 16;
   constexpr int NTILE = NTI * (NTI + 1) / 2;
-  const int n = c.n, mi = c.mi, m = c.m, l = c.l;
+  const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l;
+  const int rq = l >> 4, cq = l & 15;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     st.D[cc] = unit_d ? 1.0 : st.z[cc] / st.s[cc];
     if (k < mi) c.vb[k] = st.D[cc];
   }
-  WSYNC();
-  d4 acc[NTILE];
+  // tiles start as K's constant part
+  d4 T[NTILE];
 #pragma unroll
-  for (int t = 0; t < NTILE; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-  const int nt = (n + 15) >> 4;
-  const int kk = l >> 4, ii = l & 15;
-  for (int k0 = 0; k0 < mi; k0 += 4) {
-    const int k = k0 + kk;
-    const bool kv = k < mi;
-    const double dk = kv ? c.vb[k] : 0.0;
-    double g[NTI];
+  for (int I = 0; I < NTI; ++I) {
 #pragma unroll
-    for (int I = 0; I < NTI; ++I) {
-      const int cl = 16 * I + ii;
-      g[I] = (kv && cl < n) ? (double)c.Gs[k * n + cl] : 0.0;
-    }
-    int t = 0;
+    for (int J = 0; J <= I; ++J) {
 #pragma unroll
-    for (int I = 0; I < NTI; ++I) {
-      const double a = g[I] * dk;
-#pragma unroll
-      for (int J = 0; J <= I; ++J, ++t) {
-        if (I < nt) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, g[J], acc[t], 0, 0, 0);
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * I + 4 * r + rq, cl = 16 * J + cq;
+        const bool in = row < m && cl < m;
+        const double v = c.KQ[(in ? cl : 0) * m + (in ? row : 0)];
+        T[tidx(I, J)][r] = in ? v : 0.0;
       }
     }
   }
-  // Move the tiles into row-per-lane layout through LDS (staging area = the factor's region).
-  // Tile element (i, j) sits in lane j + 16*(i%4), register i/4.
-#pragma unroll
-  for (int j = 0; j < NMAX; ++j) st.kr[j] = (j < m && l < m) ? c.KQ[j * m + l] : 0.0;
-  double diag = l < m ? c.KQ[l * m + l] : 0.0;
+  WSYNC();
+  // G^T D G into the tiles, four rows of G per step, the next step's operands loaded while the
+  // current step's MFMAs run.  Columns >= n load as exact zeros.  No branch may separate the
+  // MFMAs (the accumulators would be copied around each one).
   {
-    int t = 0;
+    int off[NTI];
+    double msk[NTI];
 #pragma unroll
     for (int I = 0; I < NTI; ++I) {
-#pragma unroll
-      for (int J = 0; J <= I; ++J, ++t) {
-        if (I >= nt) continue;
-        const int jj = l & 15, i0 = l >> 4;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) c.Lp[(4 * r + i0) * 16 + jj] = acc[t][r];
-        WSYNC();
-        if ((l >> 4) == I) {
-          const int row = l & 15;
-#pragma unroll
-          for (int q2 = 0; q2 < 16; q2 += 2) {
-            const double2 v = *reinterpret_cast<const double2*>(c.Lp + row * 16 + q2);
-            if (16 * I + row < n) {
-              if (16 * J + q2 < n) st.kr[16 * J + q2] += v.x;
-              if (16 * J + q2 + 1 < n) st.kr[16 * J + q2 + 1] += v.y;
-              if (I == J) {
-                if (q2 == row) diag += v.x;
-                if (q2 + 1 == row) diag += v.y;
-              }
-            }
-          }
-        }
-        WSYNC();
-      }
+      const int cl = 16 * I + cq;
+      off[I] = cl < n ? cl : n - 1;
+      msk[I] = cl < n ? 1.0 : 0.0;
     }
-  }
-  // Quasi-definite LDL^T, right-looking, one pivot per step; rows in lanes.
-  double d0 = diag;
+    auto fetch = [&](int k0, double (&g)[NTI], double& dk) {
+      const int k = k0 + rq;
+      const int kc = k < mi ? k : mi - 1;
+      dk = k < mi ? c.vb[kc] : 0.0;
+      const TS* grow = c.Gs + kc * n;
 #pragma unroll
-  for (int j = 0; j < NMAX; ++j) {
-    if (j < m) {
-      if (j == n && l >= n) d0 = diag;
-      const double dj = rl(diag, j);
-      const double d0j = rl(d0, j);
-      const bool hp = j < n;
-      const bool ok = hp ? (dj > kPivotRel * fabs(d0j)) : (dj < -kPivotRel * fabs(d0j));
-      const double d = ok ? dj : (hp ? kHugePivot : -kHugePivot);
-      const double inv = recip(d);
-      if (l == j) st.rdpiv = inv;
-      double* cb = c.col + (j & 1) * 64;
-      const double cv = st.kr[j];
-      cb[l] = cv;
-      WSYNC();
-      // rows above the pivot (and padding rows, which are zero) take no update
-      const double lf = (l > j) ? cv * inv : 0.0;
-      diag -= lf * cv;
-      // columns k >= m hold zeros in every row, so the update runs to NMAX unguarded
+      for (int I = 0; I < NTI; ++I) g[I] = (double)grow[off[I]] * msk[I];
+    };
+    double gc[NTI], dc;
+    fetch(0, gc, dc);
+    for (int k0 = 0; k0 < mi; k0 += 4) {
+      double gn[NTI], dn;
+      fetch(k0 + 4 < mi ? k0 + 4 : k0, gn, dn);
 #pragma unroll
-      for (int k = (j + 1) & ~1; k < NMAX; k += 2) {
-        const double2 c2 = *reinterpret_cast<const double2*>(cb + k);
-        if (k > j) st.kr[k] -= lf * c2.x;
-        st.kr[k + 1] -= lf * c2.y;
+      for (int I = 0; I < NTI; ++I) {
+        const double a = gc[I] * dc;
+#pragma unroll
+        for (int J = 0; J <= I; ++J)
+          T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, gc[J], T[tidx(I, J)], 0, 0, 0);
       }
-      st.kr[j] = lf;
-      if (l > j && l < m) c.Lp[l * (l + 1) / 2 + j] = lf;
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) gc[I] = gn[I];
+      dc = dn;
     }
   }
   WSYNC();
+  double* pb = c.pb;
+  // Diagonal of the not yet eliminated part: lanes holding diagonal entries of the diagonal tiles
+  // publish them, every lane reads its row's.
+  auto snap_diag = [&](int from) {
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * r + rq == cq) pb[16 * I + cq] = T[tidx(I, I)][r];
+    }
+    WSYNC();
+    const double v = pb[l];
+    WSYNC();
+    return v;
+  };
+  double d0 = snap_diag(0);
+  // Panels of 4 pivots.
+#pragma unroll
+  for (int p = 0; p < NMAX / 4; ++p) {
+    if (4 * p >= m) continue;
+    constexpr int dummy = 0;
+    (void)dummy;
+    const int J = p >> 2, jc = 4 * (p & 3), c0 = 4 * p;
+    if (c0 == np) {  // first pivot of the Schur block: its reference diagonal
+      const double v = snap_diag(0);
+      if (l >= np) d0 = v;
+    }
+    // extract the panel's 4 columns, one row per lane
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+      if (I < J) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (cq >= jc && cq < jc + 4) pb[(16 * I + 4 * r + rq) * 4 + (cq - jc)] = T[tidx(I, J)][r];
+    }
+    WSYNC();
+    double v[4];
+    {
+      const double2 v01 = *reinterpret_cast<const double2*>(pb + l * 4);
+      const double2 v23 = *reinterpret_cast<const double2*>(pb + l * 4 + 2);
+      v[0] = v01.x; v[1] = v01.y; v[2] = v23.x; v[3] = v23.y;
+    }
+    WSYNC();
+    // the 4 pivots in sequence
+    double lf[4], cv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int cc = c0 + t;
+      if (cc < m) {
+        const double dj = rl(v[t], cc);
+        const double d0j = rl(d0, cc);
+        const bool hp = cc < np;
+        const bool ok = hp ? (dj > kPivotRel * fabs(d0j)) : (dj < -kPivotRel * fabs(d0j));
+        const double inv = recip(ok ? dj : (hp ? kHugePivot : -kHugePivot));
+        if (l == cc) st.rdpiv = inv;
+        cv[t] = v[t];
+        lf[t] = (l > cc) ? v[t] * inv : 0.0;
+#pragma unroll
+        for (int u = t + 1; u < 4; ++u) v[u] -= lf[t] * rl(v[t], c0 + u);
+      } else {
+        cv[t] = 0.0;
+        lf[t] = 0.0;
+      }
+    }
+    // multipliers into the packed factor; rank-4 trailing update through the matrix cores
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (l > c0 + t && l < m) c.Lp[l * (l + 1) / 2 + c0 + t] = lf[t];
+    *reinterpret_cast<double2*>(pb + l * 8) = double2{lf[0], lf[1]};
+    *reinterpret_cast<double2*>(pb + l * 8 + 2) = double2{lf[2], lf[3]};
+    *reinterpret_cast<double2*>(pb + l * 8 + 4) = double2{cv[0], cv[1]};
+    *reinterpret_cast<double2*>(pb + l * 8 + 6) = double2{cv[2], cv[3]};
+    WSYNC();
+    double aA[NTI], bB[NTI];
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+      aA[I] = (I >= J) ? -pb[(16 * I + cq) * 8 + rq] : 0.0;
+      bB[I] = (I >= J) ? pb[(16 * I + cq) * 8 + 4 + rq] : 0.0;
+    }
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+#pragma unroll
+      for (int Jp = 0; Jp <= I; ++Jp)
+        if (Jp >= J) T[tidx(I, Jp)] = __builtin_amdgcn_mfma_f64_16x16x4f64(aA[I], bB[Jp], T[tidx(I, Jp)], 0, 0, 0);
+    }
+    WSYNC();
+  }
 }
 
-// Solve K [dx; dy] = [p1; v4] for the cvxopt-reduced system of qpax/pdip.py:49-63 with
-// v1 = sg*a1 (augmented with v4 = sg*a4 in lanes n..m-1), v2 = sg*a2, v3 = sg*a3.
-// Writes st.da (dx; dy) and the given ds/dz chunks.
+// Solve K [dx; dy] = [p1; v4] (qpax/pdip.py:49-63) with v1 = sg*a1 (augmented with v4 = sg*a4 in
+// rows np..m-1: the argument pa), v2 = sg*a2, v3 = sg*a3.  Writes st.da (dx; 0; dy) and the given
+// ds/dz chunks.
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double pa,
                                       const double (&a2)[MIC], const double (&a3)[MIC], double sg,
@@ -328,18 +394,26 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   WSYNC();
   if (l < n) pa += gcol_dot(c.Gs, n, mi, l, c.vb);
   if (l >= m) pa = 0.0;
-  // forward: unit-lower L y = p; kr[j] is zero in rows <= j
+  // forward: unit-lower L y = p; lane l reads row l of L (rows >= m read row m-1, unused)
+  const int lr = l < m ? l : m - 1;
+  const double* Lrow = c.Lp + lr * (lr + 1) / 2;
 #pragma unroll
   for (int j = 0; j < NMAX; ++j) {
-    if (j < m) pa -= st.kr[j] * rl(pa, j);
+    if (j < m) {
+      const double lv = Lrow[j < lr ? j : 0];
+      pa -= (l > j ? lv : 0.0) * rl(pa, j);
+    }
   }
   pa *= st.rdpiv;
-  // backward: L^T x = y, rows of L from the packed copy in LDS.  The read for lanes >= j runs
-  // past row j into the next rows / the broadcast buffers that follow the factor (in bounds of
-  // this wavefront's LDS image) and is masked off.
-  for (int j = m - 1; j > 0; --j) {
-    const double lv = c.Lp[j * (j + 1) / 2 + l];
-    pa -= (l < j ? lv : 0.0) * rl(pa, j);
+  // backward: L^T x = y, column j of L^T is row j of L.  For lanes >= j the read runs past row j
+  // into the following rows / the panel buffer that follows the factor (in bounds of this
+  // wavefront's LDS image) and is masked off.
+#pragma unroll
+  for (int j = NMAX - 1; j > 0; --j) {
+    if (j < m) {
+      const double lv = c.Lp[j * (j + 1) / 2 + l];
+      pa -= (l < j ? lv : 0.0) * rl(pa, j);
+    }
   }
   st.da = l < m ? pa : 0.0;
   c.col[l] = st.da;
@@ -374,13 +448,13 @@ __device__ __forceinline__ double ort(const double (&x)[MIC], const double (&dx)
 
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void init(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st) {
-  const int n = c.n, mi = c.mi, m = c.m, l = c.l;
+  const int mi = c.mi, l = c.l;
   factor(c, st, true);
   // p = [-q + G^T h; b]: with a2 = 0, a3 = h, sg = 1 the solve forms p1 = v1 + G^T h
   double zero[MIC];
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;
-  const double pa = l < n ? -st.qa : (l < m ? -st.qa : 0.0);  // -q  and  b (qa holds -b)
+  const double pa = -st.qa;  // -q in rows < n, b in rows np..m-1 (qa holds -b), 0 elsewhere
   solve(c, st, pa, zero, st.h, 1.0, st.ds, st.dz);
   st.xa = st.da;
   // z = G x - h  ==  -(h - G x) = -ds from the solve above
@@ -460,12 +534,12 @@ __device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, do
   extern __shared__ __align__(16) unsigned char qpx_smem[];
   const Lds w = lds_layout(n, me, mi, (int)sizeof(TS));
   Ctx<TS, NMAX, MIC> c;
-  c.n = n; c.me = me; c.mi = mi; c.m = n + me; c.l = threadIdx.x;
+  c.n = n; c.np = padded_n(n); c.me = me; c.mi = mi; c.m = c.np + me; c.l = threadIdx.x;
   c.Gs = reinterpret_cast<const TS*>(qpx_smem + w.off_G);
-  c.As = reinterpret_cast<const TS*>(qpx_smem + w.off_A);
   c.Lp = reinterpret_cast<double*>(qpx_smem + w.off_L);
-  c.col = reinterpret_cast<double*>(qpx_smem + w.off_c);
-  c.vb = reinterpret_cast<double*>(qpx_smem + w.off_v);
+  c.pb = reinterpret_cast<double*>(qpx_smem + w.off_P);
+  c.col = c.pb;
+  c.vb = c.pb + 128;
   c.KQ = ws;
   return c;
 }
@@ -473,9 +547,9 @@ __device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, do
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void store_state(const Ctx<TS, NMAX, MIC>& c, const St<TS, NMAX, MIC>& st, int64_t p,
                                             TS* x, TS* s, TS* z, TS* y, bool post) {
-  const int n = c.n, me = c.me, mi = c.mi, l = c.l;
+  const int n = c.n, np = c.np, me = c.me, mi = c.mi, l = c.l;
   if (l < n) x[p * n + l] = (TS)st.xa;
-  else if (l < n + me) y[p * me + (l - n)] = (TS)st.xa;
+  else if (l >= np && l < np + me) y[p * me + (l - np)] = (TS)st.xa;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
@@ -522,12 +596,12 @@ __global__ __launch_bounds__(64) void k_wave_solve(Args<TS> a, int with_relax) {
 template <typename TS, int NMAX, int MIC>
 __global__ __launch_bounds__(64) void k_wave_bwd(Args<TS> a) {
   const int64_t p = blockIdx.x;
-  const int n = a.n, me = a.me, mi = a.mi, m = n + me;
+  const int n = a.n, me = a.me, mi = a.mi;
   const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + p * a.stride);
-  const int l = c.l;
+  const int l = c.l, np = c.np, m = c.m;
   St<TS, NMAX, MIC> st;
   load(c, st, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me, a.G + p * mi * n, a.h + p * mi);
-  st.xa = l < n ? (double)a.xr_in[p * n + l] : (l < m ? (double)a.yr_in[p * me + (l - n)] : 0.0);
+  st.xa = l < n ? (double)a.xr_in[p * n + l] : ((l >= np && l < m) ? (double)a.yr_in[p * me + (l - np)] : 0.0);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
@@ -541,13 +615,17 @@ __global__ __launch_bounds__(64) void k_wave_bwd(Args<TS> a) {
   const double g = l < n ? (double)a.g[p * n + l] : 0.0;
   solve(c, st, -g, zero, zero, 1.0, st.ds, st.dz);
   // cotangents; dz holds dlam_tilde, dlam = dlam_tilde / lam
+  double* lamv = c.vb;        // lam = z
+  double* dlamv = c.vb + mi;  // dlam = dlam_tilde / lam
   c.col[l] = st.xa;
   c.col[64 + l] = st.da;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
-    if (k < mi) c.vb[k] = st.z[cc];
-    if (k < mi) c.Lp[k] = st.dz[cc] / st.z[cc];
+    if (k < mi) {
+      lamv[k] = st.z[cc];
+      dlamv[k] = st.dz[cc] / st.z[cc];
+    }
   }
   WSYNC();
   const double* X = c.col;
@@ -558,12 +636,12 @@ __global__ __launch_bounds__(64) void k_wave_bwd(Args<TS> a) {
   }
   if (l < n) a.dq[p * n + l] = (TS)st.da;
   for (int cix = 0; cix < me; ++cix) {
-    const double dnu = DX[n + cix], nu = X[n + cix];
+    const double dnu = DX[np + cix], nu = X[np + cix];
     for (int j = l; j < n; j += 64) a.dA[(p * me + cix) * n + j] = (TS)(dnu * X[j] + nu * DX[j]);
   }
-  if (l >= n && l < m) a.db[p * me + (l - n)] = (TS)(-st.da);
+  if (l >= np && l < m) a.db[p * me + (l - np)] = (TS)(-st.da);
   for (int k = 0; k < mi; ++k) {
-    const double lam = c.vb[k], dlam = c.Lp[k];
+    const double lam = lamv[k], dlam = dlamv[k];
     for (int j = l; j < n; j += 64) a.dG[(p * mi + k) * n + j] = (TS)(lam * (dlam * X[j] + lam * DX[j]));
   }
 #pragma unroll

@@ -9,21 +9,22 @@ namespace qpx {
 namespace wave {
 
 struct Lds {
-  int off_G, off_A, off_L, off_c, off_v, bytes;
+  int off_G, off_L, off_P, bytes;
 };
 __host__ __device__ inline int al16(int v) { return (v + 15) & ~15; }
-// LDS carve-up of one wavefront's problem: G and A in input precision, the packed factor (at
-// least 256 doubles: it doubles as the G^T D G tile staging area), two 64-entry broadcast
-// buffers, one m_ineq-entry broadcast buffer.
+// Rows n..np-1 of the reduced KKT system are padding so that the equality block starts at a
+// multiple of 4 (the factorisation eliminates 4 pivots per panel).
+__host__ __device__ inline int padded_n(int n) { return (n + 3) & ~3; }
+// LDS carve-up of one wavefront's problem: G in input precision, the packed unit-lower factor of
+// the m x m reduced KKT matrix (m = padded_n(n) + m_eq), and a panel buffer of 64 x 8 doubles that
+// doubles as the broadcast buffers (128 + 2 m_ineq doubles) outside the factorisation.
 __host__ __device__ inline Lds lds_layout(int n, int me, int mi, int ts) {
-  const int m = n + me;
+  const int m = padded_n(n) + me;
   Lds w;
   int o = 0;
   w.off_G = o; o = al16(o + mi * n * ts);
-  w.off_A = o; o = al16(o + me * n * ts);
-  w.off_L = o; o = al16(o + 8 * (m * (m + 1) / 2 > 256 ? m * (m + 1) / 2 : 256));
-  w.off_c = o; o = al16(o + 8 * 128);
-  w.off_v = o; o = al16(o + 8 * (mi > 64 ? mi : 64));
+  w.off_L = o; o = al16(o + 8 * (m * (m + 1) / 2));
+  w.off_P = o; o = al16(o + 8 * (128 + 2 * mi > 512 ? 128 + 2 * mi : 512));
   w.bytes = o;
   return w;
 }

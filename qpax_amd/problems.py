@@ -67,6 +67,29 @@ def planted_kkt(B, nx, ns, ny, seed=0):
     return Q, q, A, b, G, h, x, s, z, y
 
 
+def planted_kkt_legacy(nx, ns, ny, rs):
+    """One problem drawn exactly as test/misc_test_utils.py:5-29 draws it from the legacy numpy
+    generator ``rs`` (a numpy.random.RandomState), so a seeded sequence reproduces the
+    reference's test problems.  Returns (Q, q, A, b, G, h, x, s, z, y)."""
+    x = rs.randn(nx)
+    M = rs.randn(nx, nx)
+    Q = M.T @ M
+    G = rs.randn(ns, nx)
+    A = rs.randn(ny, nx)
+    s = np.abs(rs.randn(ns))
+    z = np.abs(rs.randn(ns))
+    for i in range(ns):
+        if rs.rand() < 0.5:
+            s[i] = 0
+        else:
+            z[i] = 0
+    h = G @ x + s
+    b = A @ x
+    y = rs.randn(ny)
+    q = -Q @ x - G.T @ z - A.T @ y
+    return Q, q, A, b, G, h, x, s, z, y
+
+
 def uneven(B, seed=4, n_lo=8, n_hi=128):
     """B problems with n ~ U{n_lo..n_hi}, m_ineq ~ U{n..3n}, m_eq ~ U{0..n//4}.  Returns a list
     of (Q, q, A, b, G, h) tuples of unbatched arrays."""
