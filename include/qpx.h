@@ -40,6 +40,13 @@ typedef enum qpx_status {
 
 enum { QPX_F32 = 0, QPX_F64 = 1 };
 enum { QPX_PLAIN = 0, QPX_FUSED = 1 };
+/* flags of the backward and relaxation entry points */
+enum {
+  QPX_EXACT_DG = 1,           /* dG = diag(lam) dlam x^T + lam dx^T, the exact VJP (default: the
+                                 reference's diag(lam) (dlam x^T + lam dx^T), qpax/diff_qp.py:11) */
+  QPX_ELASTIC_CONSISTENT = 2  /* elastic relaxation with the forward's signs of t and r2 (default:
+                                 the reference's flipped signs, qpax/elastic_qp.py:257-261) */
+};
 
 /* Library version (major*10000 + minor*100 + patch). */
 QPX_API int32_t qpx_version(void);
@@ -79,14 +86,55 @@ QPX_API qpx_status qpx_solve_qp_relaxed(int32_t dtype, int32_t mode, int64_t bat
 /* Backward of qpax.solve_qp_primal -- replaces solve_qp_primal_backward / diff_qp /
  * optnet_derivatives (qpax/diff_qp.py:8-46, :74-79): factor the KKT system at the relaxed point
  * (sr, zr), solve it with right-hand side (-dl_dx, 0, 0, 0) and form the six cotangents
- * dQ (B,n,n), dq (B,n), dA (B,m_eq,n), db (B,m_eq), dG (B,m_ineq,n), dh (B,m_ineq). */
+ * dQ (B,n,n), dq (B,n), dA (B,m_eq,n), db (B,m_eq), dG (B,m_ineq,n), dh (B,m_ineq).
+ * flags: 0 or QPX_EXACT_DG. */
 QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
                                int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
                                const void* A, const void* b, const void* G, const void* h,
                                const void* xr, const void* sr, const void* zr, const void* yr,
-                               const void* dl_dx, void* dQ, void* dq, void* dA, void* db,
-                               void* dG, void* dh, void* workspace, size_t workspace_bytes,
-                               void* stream);
+                               const void* dl_dx, int32_t flags, void* dQ, void* dq, void* dA,
+                               void* db, void* dG, void* dh, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/* Elastic QP:  minimise 1/2 x'Qx + q'x + penalty 1't  s.t.  G x - t <= h, t >= 0 [, A x = b].
+ * qpx_solve_qp_elastic replaces qpax.solve_qp_elastic (qpax/elastic_qp.py:184-228: the
+ * initialisation :9-47 and the predictor-corrector loop of Algorithm 4 :99-174 with the
+ * one-factorisation reduction of Algorithm 5 :50-96).  penalty is a device array of `batch`
+ * values.  The equality rows (m_eq > 0: A, b in, y out) are an extension; the reference's elastic
+ * functions take none (pass m_eq = 0 for the reference's API).
+ * Outputs: x (B,n), t, s1, s2, z1, z2 (B,m_ineq), y (B,m_eq), converged, iters (B). */
+QPX_API qpx_status qpx_solve_qp_elastic(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
+                                        int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
+                                        const void* A, const void* b, const void* G, const void* h,
+                                        const void* penalty, double solver_tol, void* x, void* t,
+                                        void* s1, void* s2, void* z1, void* z2, void* y,
+                                        int32_t* converged, int32_t* iters, void* workspace,
+                                        size_t workspace_bytes, void* stream);
+
+/* Forward of qpax.solve_qp_elastic_primal under differentiation -- replaces
+ * solve_qp_elastic_primal_fwd (qpax/elastic_qp.py:433-456): solve, then relax_qp_elastic
+ * (:319-359) from the solution.  flags: 0 or QPX_ELASTIC_CONSISTENT. */
+QPX_API qpx_status qpx_solve_qp_elastic_relaxed(
+    int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+    const void* Q, const void* q, const void* A, const void* b, const void* G, const void* h,
+    const void* penalty, double solver_tol, double target_kappa, int32_t flags, void* x, void* t,
+    void* s1, void* s2, void* z1, void* z2, void* y, int32_t* converged, int32_t* iters, void* xr,
+    void* tr, void* s1r, void* s2r, void* z1r, void* z2r, void* yr, int32_t* relax_converged,
+    int32_t* relax_iters, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Backward of qpax.solve_qp_elastic_primal -- replaces solve_qp_elastic_primal_bwd /
+ * diff_qp_elastic / optnet_derivatives_elastic (qpax/elastic_qp.py:366-417, :459-466):
+ * cotangents dQ, dq, dG, dh (and dA, db with equality rows); none for the penalty.
+ * flags: 0 or QPX_EXACT_DG. */
+QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
+                                       int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
+                                       const void* A, const void* b, const void* G, const void* h,
+                                       const void* penalty, const void* xr, const void* tr,
+                                       const void* s1r, const void* s2r, const void* z1r,
+                                       const void* z2r, const void* yr, const void* dl_dx,
+                                       int32_t flags, void* dQ, void* dq, void* dA, void* db,
+                                       void* dG, void* dh, void* workspace, size_t workspace_bytes,
+                                       void* stream);
 
 #ifdef __cplusplus
 }

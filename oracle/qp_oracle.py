@@ -319,185 +319,268 @@ def solve_qp_primal_bwd(Q, q, A, b, G, h, relaxed, dl_dx, exact_dG=False):
 # ----------------------------------------------------------------------------------------------
 # elastic QP (qpax/elastic_qp.py)
 # ----------------------------------------------------------------------------------------------
-def solve_init_elastic_ls(Q, q, G, h, penalty):
+def _aug_solve(H, A, r1, r4):
+    """Solve [[H, A^T], [A, 0]] [x; y] = [r1; r4] the reference way: QR of H and of the Schur
+    complement F = A H^{-1} A^T (qpax/pdip.py:17-24).  With no rows in A this is qr_solve(H, r1)."""
+    L_H = qr(H)
+    if A is None or A.shape[-2] == 0:
+        return qr_solve(L_H, r1), None, L_H, None
+    F = np.matmul(A, qr_solve(L_H, np.swapaxes(A, -1, -2)))
+    L_F = qr(F)
+    y = qr_solve(L_F, _mv(A, qr_solve(L_H, r1)) - r4)
+    x = qr_solve(L_H, r1 - _mtv(A, y))
+    return x, y, L_H, L_F
+
+
+def _aug_solve_factored(L_H, L_F, A, r1, r4):
+    if L_F is None:
+        return qr_solve(L_H, r1), None
+    y = qr_solve(L_F, _mv(A, qr_solve(L_H, r1)) - r4)
+    return qr_solve(L_H, r1 - _mtv(A, y)), y
+
+
+# Equality rows for the elastic QP are an extension: qpax's elastic functions have no A, b
+# (qpax/elastic_qp.py:184,421).  With A = None every function below restates the reference
+# exactly; with equality rows the one n x n solve of Algorithm 5 becomes the reduced KKT solve of
+# [[H, A^T], [A, 0]] (as the standard form does), stationarity gains A^T y and the residuals gain
+# A x - b.  The extension is checked against the expanded standard-form problem.
+def solve_init_elastic_ls(Q, q, G, h, penalty, A=None, b=None):
     """qpax/elastic_qp.py:9-24."""
     ns = h.shape[-1]
+    pen = penalty if np.ndim(penalty) == 0 else np.asarray(penalty).reshape(-1, 1)
     r1 = -q
-    r2 = penalty * np.ones((h.shape[0], ns), h.dtype)
+    r2 = pen * np.ones((h.shape[0], ns), h.dtype)
     r4 = h
-    L_H = qr(Q + 0.5 * np.matmul(np.swapaxes(G, -1, -2), G))
-    x = qr_solve(L_H, r1 - 0.5 * _mtv(G, r2 - r4))
+    H = Q + 0.5 * np.matmul(np.swapaxes(G, -1, -2), G)
+    x, y, _, _ = _aug_solve(H, A, r1 - 0.5 * _mtv(G, r2 - r4), b)
     z2 = 0.5 * (_mv(G, x) + r2 - r4)
     z1 = r2 - z2
     t = -z1
-    return np.concatenate((x, t), -1), np.concatenate((z1, z2), -1)
+    return np.concatenate((x, t), -1), np.concatenate((z1, z2), -1), y
 
 
-def initialize_elastic(Q, q, G, h, penalty):
-    """qpax/elastic_qp.py:27-47 -- least-squares init plus the CVXOPT shift on stacked (z1, z2)."""
-    x_big, z_big = solve_init_elastic_ls(Q, q, G, h, penalty)
+def initialize_elastic(Q, q, G, h, penalty, A=None, b=None):
+    """qpax/elastic_qp.py:27-47 -- least-squares init plus the CVXOPT shift on stacked (z1, z2).
+    Returns (x, t, s1, s2, z1, z2) (and y when equality rows are given)."""
+    x_big, z_big, y = solve_init_elastic_ls(Q, q, G, h, penalty, A, b)
     alpha_p = -np.min(-z_big, axis=-1, keepdims=True)
     s_big = np.where(alpha_p < 0, -z_big, -z_big + (1 + alpha_p))
     alpha_d = -np.min(z_big, axis=-1, keepdims=True)
     z_big = np.where(alpha_d >= 0, z_big + (1 + alpha_d), z_big)
     nx = q.shape[-1]
     ns = h.shape[-1]
-    return (x_big[:, :nx], x_big[:, nx:], s_big[:, :ns], s_big[:, ns:], z_big[:, :ns], z_big[:, ns:])
+    out = (x_big[:, :nx], x_big[:, nx:], s_big[:, :ns], s_big[:, ns:], z_big[:, :ns], z_big[:, ns:])
+    return out if y is None else (*out, y)
 
 
-def solve_elastic_kkt_affine(s1, z1, s2, z2, Q, G, r1, r2, r3, r4, r5, r6, L_H=None):
+def solve_elastic_kkt_affine(s1, z1, s2, z2, Q, G, r1, r2, r3, r4, r5, r6, L_H=None, A=None, r7=None, L_F=None):
     """qpax/elastic_qp.py:50-96 -- Algorithm 5 (one n x n factorisation); with L_H given this is
-    solve_elastic_kkt_cc (the corrector reuses the factor)."""
+    solve_elastic_kkt_cc (the corrector reuses the factor).  Returns dx, dt, ds1, ds2, dz1, dz2,
+    (L_H, L_F) and dy (None without equality rows)."""
     a1 = s1 / z1
     a2 = s2 / z2
     w1 = r3 / z1
     w2 = r4 / z2
     p1 = r5 - r6 + w2 - w1 - a1 * r2
     a3 = a1 + a2
+    rhs = r1 - _mtv(G, p1 / a3)
     if L_H is None:
         H = Q + np.matmul(np.swapaxes(G, -1, -2), G * (1 / a3)[..., None])
-        L_H = qr(H)
-    dx = qr_solve(L_H, r1 - _mtv(G, p1 / a3))
+        dx, dy, L_H, L_F = _aug_solve(H, A, rhs, r7)
+    else:
+        dx, dy = _aug_solve_factored(L_H, L_F, A, rhs, r7)
     dz2 = (p1 + _mv(G, dx)) / a3
     dz1 = -r2 - dz2
     ds1 = (r3 - s1 * dz1) / z1
     ds2 = (r4 - s2 * dz2) / z2
     dt = ds1 - r5
-    return dx, dt, ds1, ds2, dz1, dz2, L_H
+    return dx, dt, ds1, ds2, dz1, dz2, (L_H, L_F), dy
 
 
-def pdip_pc_step_elastic(Q, q, G, h, penalty, x, t, s1, s2, z1, z2, solver_tol):
+def _elastic_res(Q, q, G, h, pen, x, t, s1, s2, z1, z2, kappa, sign_t, sign_r2, A, b, y):
+    r1 = _mv(Q, x) + q + _mtv(G, z2)
+    if A is not None:
+        r1 = r1 + _mtv(A, y)
+    r2 = sign_r2 * (-z1 - z2 + pen)
+    r3 = s1 * z1 - kappa
+    r4 = s2 * z2 - kappa
+    r5 = sign_t * t + s1
+    r6 = _mv(G, x) + sign_t * t + s2 - h
+    r7 = None if A is None else _mv(A, x) - b
+    return r1, r2, r3, r4, r5, r6, r7
+
+
+def pdip_pc_step_elastic(Q, q, G, h, penalty, x, t, s1, s2, z1, z2, solver_tol, A=None, b=None, y=None):
     """qpax/elastic_qp.py:99-174 -- Algorithm 4, one predictor-corrector step."""
     pen = penalty[..., None] if np.ndim(penalty) else penalty
-    r1 = _mv(Q, x) + q + _mtv(G, z2)
-    r2 = -z1 - z2 + pen
-    r3 = s1 * z1
-    r4 = s2 * z2
-    r5 = -t + s1
-    r6 = _mv(G, x) - t + s2 - h
-    converged = np.where(_inf_norm(r1, r2, r3, r4, r5, r6) < solver_tol, 1, 0).astype(np.int32)
-
-    _, _, ds1_a, ds2_a, dz1_a, dz2_a, L_H = solve_elastic_kkt_affine(
-        s1, z1, s2, z2, Q, G, -r1, -r2, -r3, -r4, -r5, -r6)
+    r1, r2, r3, r4, r5, r6, r7 = _elastic_res(Q, q, G, h, pen, x, t, s1, s2, z1, z2, 0.0, -1.0, 1.0, A, b, y)
+    norms = (r1, r2, r3, r4, r5, r6) if r7 is None else (r1, r2, r3, r4, r5, r6, r7)
+    converged = np.where(_inf_norm(*norms) < solver_tol, 1, 0).astype(np.int32)
+    m7 = None if r7 is None else -r7
+    _, _, ds1_a, ds2_a, dz1_a, dz2_a, (L_H, L_F), _ = solve_elastic_kkt_affine(
+        s1, z1, s2, z2, Q, G, -r1, -r2, -r3, -r4, -r5, -r6, A=A, r7=m7)
     s = np.concatenate((s1, s2), -1)
     z = np.concatenate((z1, z2), -1)
     sigma, mu = centering_params(s, z, np.concatenate((ds1_a, ds2_a), -1), np.concatenate((dz1_a, dz2_a), -1))
     sm = (sigma * mu)[..., None]
     r3 = r3 - (sm - ds1_a * dz1_a)
     r4 = r4 - (sm - ds2_a * dz2_a)
-    dx, dt, ds1, ds2, dz1, dz2, _ = solve_elastic_kkt_affine(
-        s1, z1, s2, z2, Q, G, -r1, -r2, -r3, -r4, -r5, -r6, L_H=L_H)
+    dx, dt, ds1, ds2, dz1, dz2, _, dy = solve_elastic_kkt_affine(
+        s1, z1, s2, z2, Q, G, -r1, -r2, -r3, -r4, -r5, -r6, L_H=L_H, A=A, r7=m7, L_F=L_F)
     ds = np.concatenate((ds1, ds2), -1)
     dz = np.concatenate((dz1, dz2), -1)
     alpha = 0.99 * np.minimum(np.minimum(1.0, 0.99 * ort_linesearch(s, ds)), 0.99 * ort_linesearch(z, dz))
     a = alpha[..., None]
-    return x + a * dx, t + a * dt, s1 + a * ds1, s2 + a * ds2, z1 + a * dz1, z2 + a * dz2, converged
+    out = [x + a * dx, t + a * dt, s1 + a * ds1, s2 + a * ds2, z1 + a * dz1, z2 + a * dz2]
+    if A is not None:
+        out.append(y + a * dy)
+    return (*out, converged)
 
 
-def solve_qp_elastic(Q, q, G, h, penalty, solver_tol=1e-3):
-    """qpax/elastic_qp.py:184-228 -> (x, t, s1, s2, z1, z2, converged, iters)."""
+def _eq_batch(A, b, B, n):
+    if A is None:
+        return None, None
+    A = np.asarray(A).reshape(B, -1, n)
+    b = np.asarray(b).reshape(B, -1)
+    if A.shape[1] == 0:
+        return None, None
+    return A, b
+
+
+def solve_qp_elastic(Q, q, G, h, penalty, solver_tol=1e-3, A=None, b=None):
+    """qpax/elastic_qp.py:184-228 -> (x, t, s1, s2, z1, z2, converged, iters) (y is appended
+    after z2 when equality rows are given)."""
     Q = np.asarray(Q)
     q = np.asarray(q)
     B, n = q.shape
     G = np.asarray(G).reshape(B, -1, n)
     h = np.asarray(h).reshape(B, -1)
+    A, b = _eq_batch(A, b, B, n)
     pen = np.broadcast_to(np.asarray(penalty, q.dtype), (B,)).copy()
     Q = 0.5 * (Q + np.swapaxes(Q, -1, -2))
-    init = initialize_elastic(Q, q, G, h, pen[:, None])
+    init = initialize_elastic(Q, q, G, h, pen[:, None], A, b)
 
-    def step(Q, q, G, h, pen, *st):
-        return pdip_pc_step_elastic(Q, q, G, h, pen, *st, solver_tol)
-
-    st, conv, iters = _masked_loop(step, list(init), [Q, q, G, h, pen])
+    if A is None:
+        def step(Q, q, G, h, pen, *st):
+            return pdip_pc_step_elastic(Q, q, G, h, pen, *st, solver_tol)
+        st, conv, iters = _masked_loop(step, list(init), [Q, q, G, h, pen])
+    else:
+        def step(Q, q, G, h, pen, A, b, *st):
+            return pdip_pc_step_elastic(Q, q, G, h, pen, *st[:6], solver_tol, A=A, b=b, y=st[6])
+        st, conv, iters = _masked_loop(step, list(init), [Q, q, G, h, pen, A, b])
     return (*st, conv, iters)
 
 
 def pdip_newton_step_elastic(Q, q, G, h, penalty, x, t, s1, s2, z1, z2, solver_tol, target_kappa,
-                             consistent=False):
+                             consistent=False, A=None, b=None, y=None):
     """qpax/elastic_qp.py:235-316.  The reference's relaxed residuals flip the sign of t and of r2
     relative to the forward residuals (elastic_qp.py:257,260-261) while reusing the forward linear
     solve; ``consistent=False`` restates that exactly.  ``consistent=True`` uses the forward signs
-    (elastic_qp.py:105-110) with kappa subtracted from r3, r4 -- the variant the GPU library runs."""
+    (elastic_qp.py:105-110) with kappa subtracted from r3, r4."""
     pen = penalty[..., None] if np.ndim(penalty) else penalty
-    r1 = _mv(Q, x) + q + _mtv(G, z2)
-    r3 = s1 * z1 - target_kappa
-    r4 = s2 * z2 - target_kappa
-    if consistent:
-        r2 = -z1 - z2 + pen
-        r5 = -t + s1
-        r6 = _mv(G, x) - t + s2 - h
-    else:
-        r2 = z1 + z2 - pen
-        r5 = t + s1
-        r6 = _mv(G, x) + t + s2 - h
-    converged = np.where(_inf_norm(r1, r2, r3, r4, r5, r6) < solver_tol, 1, 0).astype(np.int32)
-    dx, dt, ds1, ds2, dz1, dz2, _ = solve_elastic_kkt_affine(s1, z1, s2, z2, Q, G, -r1, -r2, -r3, -r4, -r5, -r6)
+    sign = -1.0 if consistent else 1.0
+    r1, r2, r3, r4, r5, r6, r7 = _elastic_res(Q, q, G, h, pen, x, t, s1, s2, z1, z2, target_kappa, sign,
+                                              1.0 if consistent else -1.0, A, b, y)
+    norms = (r1, r2, r3, r4, r5, r6) if r7 is None else (r1, r2, r3, r4, r5, r6, r7)
+    converged = np.where(_inf_norm(*norms) < solver_tol, 1, 0).astype(np.int32)
+    m7 = None if r7 is None else -r7
+    dx, dt, ds1, ds2, dz1, dz2, _, dy = solve_elastic_kkt_affine(
+        s1, z1, s2, z2, Q, G, -r1, -r2, -r3, -r4, -r5, -r6, A=A, r7=m7)
     s = np.concatenate((s1, s2), -1)
     z = np.concatenate((z1, z2), -1)
     ds = np.concatenate((ds1, ds2), -1)
     dz = np.concatenate((dz1, dz2), -1)
     alpha = 0.99 * np.minimum(np.minimum(1.0, 0.99 * ort_linesearch(s, ds)), 0.99 * ort_linesearch(z, dz))
     a = alpha[..., None]
-    return x + a * dx, t + a * dt, s1 + a * ds1, s2 + a * ds2, z1 + a * dz1, z2 + a * dz2, converged
+    out = [x + a * dx, t + a * dt, s1 + a * ds1, s2 + a * ds2, z1 + a * dz1, z2 + a * dz2]
+    if A is not None:
+        out.append(y + a * dy)
+    return (*out, converged)
 
 
 def relax_qp_elastic(Q, q, G, h, penalty, x, t, s1, s2, z1, z2, solver_tol=1e-3, target_kappa=1e-3,
-                     consistent=False):
+                     consistent=False, A=None, b=None, y=None):
     """qpax/elastic_qp.py:319-359."""
     B = q.shape[0]
     pen = np.broadcast_to(np.asarray(penalty, q.dtype), (B,)).copy()
-
-    def step(Q, q, G, h, pen, *st):
-        return pdip_newton_step_elastic(Q, q, G, h, pen, *st, solver_tol, target_kappa, consistent)
-
-    st, conv, iters = _masked_loop(step, [x, t, s1, s2, z1, z2], [Q, q, G, h, pen])
+    if A is None:
+        def step(Q, q, G, h, pen, *st):
+            return pdip_newton_step_elastic(Q, q, G, h, pen, *st, solver_tol, target_kappa, consistent)
+        st, conv, iters = _masked_loop(step, [x, t, s1, s2, z1, z2], [Q, q, G, h, pen])
+    else:
+        def step(Q, q, G, h, pen, A, b, *st):
+            return pdip_newton_step_elastic(Q, q, G, h, pen, *st[:6], solver_tol, target_kappa, consistent,
+                                            A=A, b=b, y=st[6])
+        st, conv, iters = _masked_loop(step, [x, t, s1, s2, z1, z2, y], [Q, q, G, h, pen, A, b])
     return (*st, conv, iters)
 
 
-def optnet_derivatives_elastic(dz, dlam, z, lam):
-    """qpax/elastic_qp.py:366-373."""
+def optnet_derivatives_elastic(dz, dlam, z, lam, exact_dG=False):
+    """qpax/elastic_qp.py:366-373 (exact_dG: see optnet_derivatives)."""
     outer = lambda u, v: u[..., :, None] * v[..., None, :]  # noqa: E731
     dl_dQ = 0.5 * (outer(dz, z) + outer(z, dz))
-    dl_dG = lam[..., :, None] * (outer(dlam, z) + outer(lam, dz))
+    if exact_dG:
+        dl_dG = lam[..., :, None] * outer(dlam, z) + outer(lam, dz)
+    else:
+        dl_dG = lam[..., :, None] * (outer(dlam, z) + outer(lam, dz))
     return dl_dQ, dz, dl_dG, -lam * dlam
 
 
-def diff_qp_elastic(Q, q, G, h, x, t, s1, s2, lam1, lam2, dl_dz):
-    """qpax/elastic_qp.py:381-417 -- cotangents of (Q, q, G, h); none for the penalty."""
+def diff_qp_elastic(Q, q, G, h, x, t, s1, s2, lam1, lam2, dl_dz, exact_dG=False, A=None, y=None):
+    """qpax/elastic_qp.py:381-417 -- cotangents of (Q, q, G, h); none for the penalty.  With
+    equality rows (extension) also those of (A, b), formed as in optnet_derivatives."""
     nz = q.shape[-1]
     ns = h.shape[-1]
     zns = np.zeros_like(h)
-    dx, dt, ds1, ds2, dlam1, dlam2, _ = solve_elastic_kkt_affine(
-        s1, lam1, s2, lam2, Q, G, -dl_dz, zns, zns, zns, zns, zns)
+    r7 = None if A is None else np.zeros((q.shape[0], A.shape[-2]), q.dtype)
+    dx, dt, ds1, ds2, dlam1, dlam2, _, dnu = solve_elastic_kkt_affine(
+        s1, lam1, s2, lam2, Q, G, -dl_dz, zns, zns, zns, zns, zns, A=A, r7=r7)
     dz = np.concatenate((dx, dt), -1)
     z = np.concatenate((x, t), -1)
     lam = np.concatenate((lam1, lam2), -1)
     dlam = np.concatenate((dlam1, dlam2), -1) / lam
-    dl_dQ, dl_dq, dl_dG, dl_dh = optnet_derivatives_elastic(dz, dlam, z, lam)
-    return dl_dQ[:, :nz, :nz], dl_dq[:, :nz], dl_dG[:, ns:, :nz], dl_dh[:, ns:]
+    dl_dQ, dl_dq, dl_dG, dl_dh = optnet_derivatives_elastic(dz, dlam, z, lam, exact_dG)
+    out = (dl_dQ[:, :nz, :nz], dl_dq[:, :nz], dl_dG[:, ns:, :nz], dl_dh[:, ns:])
+    if A is None:
+        return out
+    outer = lambda u, v: u[..., :, None] * v[..., None, :]  # noqa: E731
+    return (*out, outer(dnu, x) + outer(y, dx), -dnu)
 
 
-def solve_qp_elastic_primal_fwd(Q, q, G, h, penalty, solver_tol=1e-5, target_kappa=1e-3, consistent=False):
+def solve_qp_elastic_primal_fwd(Q, q, G, h, penalty, solver_tol=1e-5, target_kappa=1e-3, consistent=False,
+                                A=None, b=None):
     """qpax/elastic_qp.py:433-456."""
-    x, t, s1, s2, z1, z2, c1, i1 = solve_qp_elastic(Q, q, G, h, penalty, solver_tol=solver_tol)
-    Qa = np.asarray(Q)
-    B, n = x.shape
+    Q = np.asarray(Q)
+    q = np.asarray(q)
+    B, n = q.shape
     Ga = np.asarray(G).reshape(B, -1, n)
     ha = np.asarray(h).reshape(B, -1)
-    rel = relax_qp_elastic(Qa, np.asarray(q), Ga, ha, penalty, x, t, s1, s2, z1, z2,
-                           solver_tol=solver_tol, target_kappa=target_kappa, consistent=consistent)
-    return x, rel[:6], (c1, i1, rel[6], rel[7])
+    Ab, bb = _eq_batch(A, b, B, n)
+    out = solve_qp_elastic(Q, q, Ga, ha, penalty, solver_tol=solver_tol, A=Ab, b=bb)
+    st = out[:-2]
+    c1, i1 = out[-2], out[-1]
+    y = st[6] if Ab is not None else None
+    rel = relax_qp_elastic(Q, q, Ga, ha, penalty, *st[:6], solver_tol=solver_tol, target_kappa=target_kappa,
+                           consistent=consistent, A=Ab, b=bb, y=y)
+    return st[0], rel[:-2], (c1, i1, rel[-2], rel[-1])
 
 
-def solve_qp_elastic_primal_bwd(Q, q, G, h, relaxed, dl_dx):
+def solve_qp_elastic_primal_bwd(Q, q, G, h, relaxed, dl_dx, exact_dG=False, A=None):
     """qpax/elastic_qp.py:459-466."""
-    xr, tr, s1r, s2r, z1r, z2r = relaxed
-    return diff_qp_elastic(np.asarray(Q), np.asarray(q), np.asarray(G), np.asarray(h),
-                           xr, tr, s1r, s2r, z1r, z2r, dl_dx)
+    Q = np.asarray(Q)
+    q = np.asarray(q)
+    B, n = q.shape
+    Ga = np.asarray(G).reshape(B, -1, n)
+    ha = np.asarray(h).reshape(B, -1)
+    Ab, _ = _eq_batch(A, np.zeros((B, 0)) if A is None else np.zeros((B, np.asarray(A).reshape(B, -1, n).shape[1])), B, n)
+    xr, tr, s1r, s2r, z1r, z2r = relaxed[:6]
+    yr = relaxed[6] if Ab is not None else None
+    return diff_qp_elastic(Q, q, Ga, ha, xr, tr, s1r, s2r, z1r, z2r, dl_dx, exact_dG, A=Ab, y=yr)
 
 
-def expanded_elastic_problem(Q, q, G, h, penalty):
-    """test/test_elastic_qp.py:39-55 -- the elastic QP written as a standard QP in (x, t)."""
+def expanded_elastic_problem(Q, q, G, h, penalty, A=None):
+    """test/test_elastic_qp.py:39-55 -- the elastic QP written as a standard QP in (x, t); with
+    equality rows A (extension) the expanded equality matrix is [A 0]."""
     B, nx = q.shape
     ns = h.shape[-1]
     pen = np.broadcast_to(np.asarray(penalty, q.dtype), (B,))
@@ -508,4 +591,7 @@ def expanded_elastic_problem(Q, q, G, h, penalty):
     G2 = np.concatenate((np.concatenate((np.zeros((B, ns, nx), q.dtype), -eye), -1),
                          np.concatenate((G, -eye), -1)), 1)
     h2 = np.concatenate((np.zeros((B, ns), q.dtype), h), -1)
-    return Q2, q2, np.zeros((B, 0, nx + ns), q.dtype), np.zeros((B, 0), q.dtype), G2, h2
+    if A is None:
+        return Q2, q2, np.zeros((B, 0, nx + ns), q.dtype), np.zeros((B, 0), q.dtype), G2, h2
+    A2 = np.concatenate((A, np.zeros((B, A.shape[1], ns), q.dtype)), -1)
+    return Q2, q2, A2, None, G2, h2

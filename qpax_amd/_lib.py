@@ -27,8 +27,16 @@ SIGNATURES = {
     "qpx_solve_qp": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_f64] + [_vp] * 6 + [_vp, _sz, _vp]),
     "qpx_solve_qp_relaxed": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_f64, _f64] + [_vp] * 12
                              + [_vp, _sz, _vp]),
-    "qpx_diff_qp": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_vp] * 5 + [_vp] * 6 + [_vp, _sz, _vp]),
+    "qpx_diff_qp": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_vp] * 5 + [_i32] + [_vp] * 6
+                    + [_vp, _sz, _vp]),
+    "qpx_solve_qp_elastic": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_f64] + [_vp] * 9
+                             + [_vp, _sz, _vp]),
+    "qpx_solve_qp_elastic_relaxed": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_f64, _f64, _i32]
+                                     + [_vp] * 18 + [_vp, _sz, _vp]),
+    "qpx_diff_qp_elastic": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_vp] * 8 + [_i32] + [_vp] * 6
+                            + [_vp, _sz, _vp]),
 }
+QPX_EXACT_DG, QPX_ELASTIC_CONSISTENT = 1, 2
 
 _lib = None
 

@@ -19,6 +19,11 @@ struct Args {
   int64_t stride;  // float64 words per problem
   unsigned* counter;
   double tol, kappa;
+  int flags;       // QPX_EXACT_DG, QPX_ELASTIC_CONSISTENT (include/qpx.h)
+  // elastic QP (s, z, sr, zr and their inputs above hold s1, z1)
+  const TS* pen;
+  TS *t, *s2, *z2, *tr, *s2r, *z2r;
+  const TS *tr_in, *s2r_in, *z2r_in;
 };
 
 }  // namespace qpx

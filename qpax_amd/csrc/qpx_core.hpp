@@ -185,12 +185,16 @@ __device__ void block_bsub_t(const double* L, const double* rcp, int m, double* 
 // --------------------------------------------------------------------------------------------
 // Stage: factorisation (qpax/pdip.py:38-46).  D = z/s; H = Q + G^T D G; H = L L^T;
 // W = L^{-1} A^T; F = W^T W = A H^{-1} A^T; F = Lf Lf^T.
-// With unit_d the scaling is D = 1 (the initialisation, pdip.py:17).
+// The scaling is D = z/s (kDzs), D = 1 (kDunit, the initialisation, pdip.py:17) or already in
+// v.D (kDgiven, used by the elastic reduction).
 // --------------------------------------------------------------------------------------------
+enum DMode : int { kDzs = 0, kDunit = 1, kDgiven = 2 };
+
 template <int NT>
-__device__ void stage_factor(const View& v, bool unit_d) {
+__device__ void stage_factor(const View& v, int dmode) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
-  for (int k = tid; k < mi; k += NT) v.D[k] = unit_d ? 1.0 : v.z[k] / v.s[k];
+  if (dmode != kDgiven)
+    for (int k = tid; k < mi; k += NT) v.D[k] = dmode == kDunit ? 1.0 : v.z[k] / v.s[k];
   __syncthreads();
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e - i * n;
@@ -377,7 +381,7 @@ __device__ void stage_step(const View& v, bool relax_rule, double* red) {
 template <int NT>
 __device__ void stage_init(const View& v, double* red) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
-  stage_factor<NT>(v, true);
+  stage_factor<NT>(v, kDunit);
   for (int i = tid; i < n; i += NT) {
     double acc = -v.q[i];
     for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * v.h[k];
@@ -410,7 +414,7 @@ __device__ void stage_init(const View& v, double* red) {
 template <int NT>
 __device__ void pc_iteration(const View& v, double tol, double* red) {
   const int conv = stage_residuals<NT>(v, 0.0, tol, red);
-  stage_factor<NT>(v, false);
+  stage_factor<NT>(v, kDzs);
   stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.dsa, v.dza);
   stage_center<NT>(v, red);
   stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
@@ -423,7 +427,7 @@ __device__ void pc_iteration(const View& v, double tol, double* red) {
 template <int NT>
 __device__ void relax_iteration(const View& v, double tol, double kappa, double* red) {
   const int conv = stage_residuals<NT>(v, kappa, tol, red);
-  stage_factor<NT>(v, false);
+  stage_factor<NT>(v, kDzs);
   stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
   stage_step<NT>(v, true, red);
   if (threadIdx.x == 0) { v.ctl[kRelaxConv] = conv; v.ctl[kRelaxIter] += 1; }

@@ -17,6 +17,7 @@
 #include "../../include/qpx.h"
 #include "qpx_args.hpp"
 #include "qpx_core.hpp"
+#include "qpx_elastic.hpp"
 
 namespace qpx {
 
@@ -69,7 +70,7 @@ template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_factor(Args<TS> a) {
   const View v = view_of(a, blockIdx.x);
   if (!v.ctl[kActive]) return;
-  stage_factor<NT>(v, false);
+  stage_factor<NT>(v, kDzs);
 }
 
 template <int NT, typename TS>
@@ -202,7 +203,9 @@ __device__ void bwd_solve_write(const Args<TS>& a, const View& v, int64_t p) {
   for (int e = tid; e < mi * n; e += NT) {
     const int k = e / n, j = e - k * n;
     const double lam = v.z[k], dlam = v.dz[k] / lam;
-    a.dG[(int64_t)p * mi * n + e] = (TS)(lam * (dlam * v.x[j] + lam * v.dx[j]));
+    // reference: diag(lam) (dlam x^T + lam dx^T) (diff_qp.py:11); exact VJP: diag(lam) dlam x^T + lam dx^T
+    const double val = (a.flags & 1) ? lam * dlam * v.x[j] + lam * v.dx[j] : lam * (dlam * v.x[j] + lam * v.dx[j]);
+    a.dG[(int64_t)p * mi * n + e] = (TS)val;
   }
   for (int k = tid; k < mi; k += NT) {
     const double lam = v.z[k], dlam = v.dz[k] / lam;
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(NT) void k_bwd_load(Args<TS> a) {
 
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_bwd_factor(Args<TS> a) {
-  stage_factor<NT>(view_of(a, blockIdx.x), false);
+  stage_factor<NT>(view_of(a, blockIdx.x), kDzs);
 }
 
 template <int NT, typename TS>
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_fused_bwd(Args<TS> a) {
   const int64_t p = blockIdx.x;
   const View v = view_of(a, p);
   bwd_load<NT, TS>(a, v, p);
-  stage_factor<NT>(v, false);
+  stage_factor<NT>(v, kDzs);
   bwd_solve_write<NT, TS>(a, v, p);
 }
 
@@ -263,7 +266,7 @@ constexpr size_t kCtlBytes = 4096;  // head of the workspace: counters
 // KKT matrix (m = n padded to a multiple of 4, plus m_eq), whichever is larger.
 int64_t stride_words(int n, int me, int mi) {
   const int64_t m = qpx::wave::padded_n(n) + me;
-  const int64_t w = qpx::view_words(n, me, mi);
+  const int64_t w = qpx::elastic_words(n, me, mi);
   return w > qpx::round_up(m * m, 64) ? w : qpx::round_up(m * m, 64);
 }
 
@@ -432,15 +435,103 @@ template <typename TS>
 qpx_status backward_impl(int mode, int64_t B, int n, int me, int mi, const void* Q, const void* q,
                          const void* A, const void* b, const void* G, const void* h,
                          const void* xr, const void* sr, const void* zr, const void* yr,
-                         const void* g, void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
+                         const void* g, int flags, void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
                          void* ws, void* stream) {
   auto a = make_args<TS>(B, n, me, mi, ws);
+  a.flags = flags;
   a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
   a.G = (const TS*)G; a.h = (const TS*)h;
   a.xr_in = (const TS*)xr; a.sr_in = (const TS*)sr; a.zr_in = (const TS*)zr; a.yr_in = (const TS*)yr;
   a.g = (const TS*)g;
   a.dQ = (TS*)dQ; a.dq = (TS*)dq; a.dA = (TS*)dA; a.db = (TS*)db; a.dG = (TS*)dG; a.dh = (TS*)dh;
   return dispatch_nt(n, mi, [&](auto nt) { return run_backward<decltype(nt)::value, TS>(a, mode, (hipStream_t)stream); });
+}
+
+// ------------------------------------ elastic -----------------------------------------------
+template <int NT, typename TS>
+qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
+  const dim3 grid((unsigned)a.B), block(NT);
+  if (a.B == 0) return QPX_OK;
+  if (mode == QPX_FUSED) {
+    hipLaunchKernelGGL((qpx::k_e_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
+    QPX_HIP(hipGetLastError());
+    return QPX_OK;
+  }
+  int* host = pinned_counter();
+  if (!host) return fail(QPX_ERR_HIP, "hipHostMalloc of the plain-mode counter failed");
+  hipLaunchKernelGGL((qpx::k_e_setup<NT, TS>), grid, block, 0, st, a);
+  QPX_HIP(hipGetLastError());
+  for (int phase = 0; phase < (relax ? 2 : 1); ++phase) {
+    const int rl = phase;
+    for (int it = 0; it < qpx::kMaxIters; ++it) {
+      hipLaunchKernelGGL((qpx::k_e_resid<NT, TS>), grid, block, 0, st, a, rl);
+      hipLaunchKernelGGL((qpx::k_e_factor<NT, TS>), grid, block, 0, st, a);
+      if (!rl) {
+        hipLaunchKernelGGL((qpx::k_e_solve<NT, TS>), grid, block, 0, st, a, 1);
+        hipLaunchKernelGGL((qpx::k_e_center<NT, TS>), grid, block, 0, st, a);
+      }
+      hipLaunchKernelGGL((qpx::k_e_solve<NT, TS>), grid, block, 0, st, a, 0);
+      hipLaunchKernelGGL((qpx::k_e_step<NT, TS>), grid, block, 0, st, a, rl);
+      QPX_HIP(hipGetLastError());
+      QPX_HIP(hipMemcpyAsync(host, a.counter, sizeof(int), hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipStreamSynchronize(st));
+      if (*host == 0) break;
+    }
+    if (!rl) hipLaunchKernelGGL((qpx::k_e_finish_solve<NT, TS>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((qpx::k_e_finish_relax<NT, TS>), grid, block, 0, st, a);
+    QPX_HIP(hipGetLastError());
+  }
+  return QPX_OK;
+}
+
+template <int NT, typename TS>
+qpx_status run_e_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
+  const dim3 grid((unsigned)a.B), block(NT);
+  if (a.B == 0) return QPX_OK;
+  if (mode == QPX_FUSED) {
+    hipLaunchKernelGGL((qpx::k_e_fused_bwd<NT, TS>), grid, block, 0, st, a);
+  } else {
+    hipLaunchKernelGGL((qpx::k_e_bwd_load<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_e_bwd_factor<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_e_bwd_solve<NT, TS>), grid, block, 0, st, a);
+  }
+  QPX_HIP(hipGetLastError());
+  return QPX_OK;
+}
+
+template <typename TS>
+qpx_status e_forward_impl(int mode, int64_t B, int n, int me, int mi, const void* Q, const void* q, const void* A,
+                          const void* b, const void* G, const void* h, const void* pen, double tol, double kappa,
+                          int flags, bool relax, void* const* out, int* conv, int* iters, void* const* rout,
+                          int* rconv, int* riters, void* ws, void* stream) {
+  auto a = make_args<TS>(B, n, me, mi, ws);
+  a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+  a.G = (const TS*)G; a.h = (const TS*)h; a.pen = (const TS*)pen;
+  a.x = (TS*)out[0]; a.t = (TS*)out[1]; a.s = (TS*)out[2]; a.s2 = (TS*)out[3];
+  a.z = (TS*)out[4]; a.z2 = (TS*)out[5]; a.y = (TS*)out[6];
+  a.conv = conv; a.iters = iters;
+  if (relax) {
+    a.xr = (TS*)rout[0]; a.tr = (TS*)rout[1]; a.sr = (TS*)rout[2]; a.s2r = (TS*)rout[3];
+    a.zr = (TS*)rout[4]; a.z2r = (TS*)rout[5]; a.yr = (TS*)rout[6];
+    a.rconv = rconv; a.riters = riters;
+  }
+  a.tol = tol; a.kappa = kappa; a.flags = flags;
+  return dispatch_nt(n, mi, [&](auto nt) { return run_e_forward<decltype(nt)::value, TS>(a, mode, relax, (hipStream_t)stream); });
+}
+
+template <typename TS>
+qpx_status e_backward_impl(int mode, int64_t B, int n, int me, int mi, const void* Q, const void* q, const void* A,
+                           const void* b, const void* G, const void* h, const void* pen, const void* const* rel,
+                           const void* g, int flags, void* const* grads, void* ws, void* stream) {
+  auto a = make_args<TS>(B, n, me, mi, ws);
+  a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+  a.G = (const TS*)G; a.h = (const TS*)h; a.pen = (const TS*)pen;
+  a.xr_in = (const TS*)rel[0]; a.tr_in = (const TS*)rel[1]; a.sr_in = (const TS*)rel[2];
+  a.s2r_in = (const TS*)rel[3]; a.zr_in = (const TS*)rel[4]; a.z2r_in = (const TS*)rel[5];
+  a.yr_in = (const TS*)rel[6]; a.g = (const TS*)g; a.flags = flags;
+  a.dQ = (TS*)grads[0]; a.dq = (TS*)grads[1]; a.dA = (TS*)grads[2]; a.db = (TS*)grads[3];
+  a.dG = (TS*)grads[4]; a.dh = (TS*)grads[5];
+  return dispatch_nt(n, mi, [&](auto nt) { return run_e_backward<decltype(nt)::value, TS>(a, mode, (hipStream_t)stream); });
 }
 
 bool any_null(std::initializer_list<const void*> ps) {
@@ -505,7 +596,7 @@ QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32
                                int32_t m_ineq, const void* Q, const void* q, const void* A,
                                const void* b, const void* G, const void* h, const void* xr,
                                const void* sr, const void* zr, const void* yr, const void* dl_dx,
-                               void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
+                               int32_t flags, void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
                                void* workspace, size_t workspace_bytes, void* stream) {
   g_err.clear();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
@@ -514,10 +605,79 @@ QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32
     return fail(QPX_ERR_ARG, "null pointer argument");
   if (batch > 0 && m_eq > 0 && any_null({A, b, yr, dA, db})) return fail(QPX_ERR_ARG, "null equality pointer");
   if (dtype == QPX_F32)
-    return backward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, dQ, dq, dA, db,
-                                dG, dh, workspace, stream);
-  return backward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, dQ, dq, dA, db,
-                               dG, dh, workspace, stream);
+    return backward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, flags, dQ, dq, dA,
+                                db, dG, dh, workspace, stream);
+  return backward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, flags, dQ, dq, dA,
+                               db, dG, dh, workspace, stream);
+}
+
+QPX_API qpx_status qpx_solve_qp_elastic(int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq,
+                                        int32_t m_ineq, const void* Q, const void* q, const void* A,
+                                        const void* b, const void* G, const void* h, const void* penalty,
+                                        double solver_tol, void* x, void* t, void* s1, void* s2, void* z1,
+                                        void* z2, void* y, int32_t* converged, int32_t* iters, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  g_err.clear();
+  qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, penalty, x, t, s1, s2, z1, z2, converged, iters, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y})) return fail(QPX_ERR_ARG, "null equality pointer");
+  void* out[7] = {x, t, s1, s2, z1, z2, y};
+  if (dtype == QPX_F32)
+    return e_forward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, solver_tol, 0.0, 0, false,
+                                 out, converged, iters, nullptr, nullptr, nullptr, workspace, stream);
+  return e_forward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, solver_tol, 0.0, 0, false,
+                                out, converged, iters, nullptr, nullptr, nullptr, workspace, stream);
+}
+
+QPX_API qpx_status qpx_solve_qp_elastic_relaxed(int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq,
+                                                int32_t m_ineq, const void* Q, const void* q, const void* A,
+                                                const void* b, const void* G, const void* h, const void* penalty,
+                                                double solver_tol, double target_kappa, int32_t flags, void* x,
+                                                void* t, void* s1, void* s2, void* z1, void* z2, void* y,
+                                                int32_t* converged, int32_t* iters, void* xr, void* tr, void* s1r,
+                                                void* s2r, void* z1r, void* z2r, void* yr, int32_t* relax_converged,
+                                                int32_t* relax_iters, void* workspace, size_t workspace_bytes,
+                                                void* stream) {
+  g_err.clear();
+  qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, penalty, x, t, s1, s2, z1, z2, converged, iters, xr, tr, s1r, s2r, z1r, z2r,
+                             relax_converged, relax_iters, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y, yr})) return fail(QPX_ERR_ARG, "null equality pointer");
+  void* out[7] = {x, t, s1, s2, z1, z2, y};
+  void* rout[7] = {xr, tr, s1r, s2r, z1r, z2r, yr};
+  if (dtype == QPX_F32)
+    return e_forward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, solver_tol, target_kappa,
+                                 flags, true, out, converged, iters, rout, relax_converged, relax_iters, workspace,
+                                 stream);
+  return e_forward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, solver_tol, target_kappa,
+                                flags, true, out, converged, iters, rout, relax_converged, relax_iters, workspace,
+                                stream);
+}
+
+QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq,
+                                       int32_t m_ineq, const void* Q, const void* q, const void* A, const void* b,
+                                       const void* G, const void* h, const void* penalty, const void* xr,
+                                       const void* tr, const void* s1r, const void* s2r, const void* z1r,
+                                       const void* z2r, const void* yr, const void* dl_dx, int32_t flags, void* dQ,
+                                       void* dq, void* dA, void* db, void* dG, void* dh, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  g_err.clear();
+  qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, penalty, xr, tr, s1r, s2r, z1r, z2r, dl_dx, dQ, dq, dG, dh, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, yr, dA, db})) return fail(QPX_ERR_ARG, "null equality pointer");
+  const void* rel[7] = {xr, tr, s1r, s2r, z1r, z2r, yr};
+  void* grads[6] = {dQ, dq, dA, db, dG, dh};
+  if (dtype == QPX_F32)
+    return e_backward_impl<float>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, rel, dl_dx, flags, grads,
+                                  workspace, stream);
+  return e_backward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, rel, dl_dx, flags, grads,
+                                 workspace, stream);
 }
 
 }  // extern "C"

@@ -640,9 +640,13 @@ __global__ __launch_bounds__(64) void k_wave_bwd(Args<TS> a) {
     for (int j = l; j < n; j += 64) a.dA[(p * me + cix) * n + j] = (TS)(dnu * X[j] + nu * DX[j]);
   }
   if (l >= np && l < m) a.db[p * me + (l - np)] = (TS)(-st.da);
+  const bool exact = a.flags & 1;
   for (int k = 0; k < mi; ++k) {
     const double lam = lamv[k], dlam = dlamv[k];
-    for (int j = l; j < n; j += 64) a.dG[(p * mi + k) * n + j] = (TS)(lam * (dlam * X[j] + lam * DX[j]));
+    for (int j = l; j < n; j += 64) {
+      const double v = exact ? lam * dlam * X[j] + lam * DX[j] : lam * (dlam * X[j] + lam * DX[j]);
+      a.dG[(p * mi + k) * n + j] = (TS)v;
+    }
   }
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {

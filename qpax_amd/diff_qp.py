@@ -16,7 +16,7 @@ from ._tensors import dtype_code, normalize, ptr, stream_of
 from .pdip import _workspace, solve_qp_batch, solve_qp_relaxed_batch
 
 
-def diff_qp_batch(bt, relaxed, dl_dx, mode="fused"):
+def diff_qp_batch(bt, relaxed, dl_dx, mode="fused", exact_dG=False):
     """Six cotangents (flat) at the relaxed point ``relaxed = (xr, sr, zr, yr)``."""
     lib = _lib.load()
     xr, sr, zr, yr = relaxed
@@ -26,7 +26,7 @@ def diff_qp_batch(bt, relaxed, dl_dx, mode="fused"):
     ws, nbytes = _workspace(bt)
     st = lib.qpx_diff_qp(dtype_code(bt.dtype), _lib.MODES[mode], bt.B, bt.n, bt.me, bt.mi,
                          *[ptr(bt[k]) for k in bt.names], ptr(xr), ptr(sr), ptr(zr), ptr(yr), ptr(g),
-                         ptr(dQ), ptr(dq), ptr(dA), ptr(db), ptr(dG), ptr(dh), ptr(ws), nbytes,
+                         _lib.QPX_EXACT_DG if exact_dG else 0, ptr(dQ), ptr(dq), ptr(dA), ptr(db), ptr(dG), ptr(dh), ptr(ws), nbytes,
                          stream_of(bt.device))
     _lib.check(st, "qpx_diff_qp")
     return dQ, dq, dA, db, dG, dh
@@ -46,7 +46,7 @@ def _reduce_to(grad, shape):
 
 class _SolveQPPrimal(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, Q, q, A, b, G, h, solver_tol, target_kappa, mode):
+    def forward(ctx, Q, q, A, b, G, h, solver_tol, target_kappa, mode, exact_dG):
         bt = normalize(Q, q, A, b, G, h)
         if bt.mi == 0:
             from .pdip import _special
@@ -57,7 +57,8 @@ class _SolveQPPrimal(torch.autograd.Function):
             out = solve_qp_relaxed_batch(bt, solver_tol, target_kappa, mode)
             x, relaxed = out[0], out[6:10]
             ctx.save_for_backward(*[bt[k] for k in bt.names], *relaxed)
-            ctx.meta = (bt.batch_shape, bt.n, bt.me, bt.mi, mode, [tuple(t.shape) for t in (Q, q, A, b, G, h)])
+            ctx.meta = (bt.batch_shape, bt.n, bt.me, bt.mi, mode, [tuple(t.shape) for t in (Q, q, A, b, G, h)],
+                        exact_dG)
         else:
             x = solve_qp_batch(bt, solver_tol, mode)[0]
         return bt.unflatten(x, bt.n)
@@ -65,17 +66,21 @@ class _SolveQPPrimal(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gx):
         Q, q, A, b, G, h, xr, sr, zr, yr = ctx.saved_tensors
-        batch_shape, n, me, mi, mode, shapes = ctx.meta
+        batch_shape, n, me, mi, mode, shapes, exact_dG = ctx.meta
         bt = normalize(Q, q, A, b, G, h)
-        grads = diff_qp_batch(bt, (xr, sr, zr, yr), gx.contiguous(), mode)
+        grads = diff_qp_batch(bt, (xr, sr, zr, yr), gx.contiguous(), mode, exact_dG)
         tails = [(n, n), (n,), (me, n), (me,), (mi, n), (mi,)]
         out = []
         for gr, tail, shp, need in zip(grads, tails, shapes, ctx.needs_input_grad[:6]):
             out.append(_reduce_to(gr.reshape(*batch_shape, *tail), shp) if need else None)
-        return (*out, None, None, None)
+        return (*out, None, None, None, None)
 
 
-def solve_qp_primal(Q, q, A, b, G, h, solver_tol: float = 1e-5, target_kappa: float = 1e-3, mode: str = "fused"):
-    """Primal solution x of the QP, differentiable w.r.t. (Q, q, A, b, G, h) (qpax/diff_qp.py:49)."""
+def solve_qp_primal(Q, q, A, b, G, h, solver_tol: float = 1e-5, target_kappa: float = 1e-3, mode: str = "fused",
+                    exact_dG: bool = False):
+    """Primal solution x of the QP, differentiable w.r.t. (Q, q, A, b, G, h) (qpax/diff_qp.py:49).
+
+    ``exact_dG=True`` returns the exact vector-Jacobian product for G; the default reproduces the
+    reference's formula (qpax/diff_qp.py:11), which scales the second term of dG by lam."""
     ts = [v if isinstance(v, torch.Tensor) else torch.as_tensor(v) for v in (Q, q, A, b, G, h)]
-    return _SolveQPPrimal.apply(*ts, solver_tol, target_kappa, mode)
+    return _SolveQPPrimal.apply(*ts, solver_tol, target_kappa, mode, exact_dG)

@@ -54,3 +54,17 @@ def test_grads_plain_agrees_with_fused():
     assert rel_err(xa, xb).max() < 1e-9
     for a, b in zip(ga, gb):
         assert rel_err(a, b).max() < 1e-6
+
+
+def test_exact_dG_matches_oracle():
+    import qpax_amd
+
+    data = [np.asarray(v, np.float64) for v in P.random_feasible(16, 50, 10, 100, seed=12)]
+    ts = [torch.tensor(v, device="cuda", requires_grad=True) for v in data]
+    x = qpax_amd.solve_qp_primal(*ts, solver_tol=1e-6, target_kappa=1e-3, exact_dG=True)
+    ((x - 1.0) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    xo, rel, info = O.solve_qp_primal_fwd(*data, solver_tol=1e-6, target_kappa=1e-3)
+    go = O.solve_qp_primal_bwd(*data, rel, 2 * (xo - 1), exact_dG=True)
+    ok = (info[0] == 1) & (info[2] == 1)
+    assert rel_err(to_np(ts[4].grad)[ok], go[4][ok]).max() < 1e-7

@@ -312,3 +312,45 @@ def test_planted_mistake_in_backward_sign_is_caught(monkeypatch):
     monkeypatch.setattr(O, "optnet_derivatives", wrong)
     with pytest.raises(AssertionError):
         test_backward_is_vjp_of_relaxed_map()
+
+
+# --------------------------- elastic equality extension ------------------------------------------
+def _elastic_eq_problem(B=6, n=8, me=2, mi=12, seed=21, shift=True):
+    Q, q, A, b, G, h = P.random_feasible(B, n, me, mi, seed=seed)
+    if shift:  # make some problems infeasible so the slacks t are active
+        h = h - 2.0 * np.abs(np.random.default_rng(seed).standard_normal(h.shape)) * (np.arange(B) % 2)[:, None]
+    return Q, q, A, b, G, h
+
+
+def test_elastic_equality_extension_matches_expanded_problem():
+    Q, q, A, b, G, h = _elastic_eq_problem()
+    pen = 5.0
+    x, t, s1, s2, z1, z2, y, c, it = O.solve_qp_elastic(Q, q, G, h, pen, solver_tol=1e-10, A=A, b=b)
+    Q2, q2, A2, _, G2, h2 = O.expanded_elastic_problem(Q, q, G, h, pen, A=A)
+    xe, se, ze, ye, ce, ie = O.solve_qp(Q2, q2, A2, b, G2, h2, solver_tol=1e-10)
+    ok = (c == 1) & (ce == 1)
+    assert ok.mean() > 0.8
+    assert np.abs(x - xe[:, :8])[ok].max() < 1e-7
+    assert np.abs(t - xe[:, 8:])[ok].max() < 1e-7
+    assert np.abs(y - ye)[ok].max() < 1e-6
+    assert np.abs(np.concatenate([z1, z2], -1) - ze)[ok].max() < 1e-6
+
+
+@pytest.mark.parametrize("consistent", [False, True])
+def test_elastic_equality_extension_grads_match_expanded(consistent):
+    Q, q, A, b, G, h = _elastic_eq_problem(seed=22)
+    pen = 5.0
+    v = np.random.default_rng(5).standard_normal((Q.shape[0], 8))
+    x, rel, info = O.solve_qp_elastic_primal_fwd(Q, q, G, h, pen, solver_tol=1e-8, target_kappa=1e-3,
+                                                 consistent=consistent, A=A, b=b)
+    ge = O.solve_qp_elastic_primal_bwd(Q, q, G, h, rel, v, A=A)
+    Q2, q2, A2, _, G2, h2 = O.expanded_elastic_problem(Q, q, G, h, pen, A=A)
+    xp, relp, infop = O.solve_qp_primal_fwd(Q2, q2, A2, b, G2, h2, solver_tol=1e-8, target_kappa=1e-3)
+    vv = np.zeros_like(xp)
+    vv[:, :8] = v
+    gp = O.solve_qp_primal_bwd(Q2, q2, A2, b, G2, h2, relp, vv)
+    ok = (info[0] == 1) & (infop[0] == 1) & (infop[2] == 1)
+    ref = (gp[0][:, :8, :8], gp[1][:, :8], gp[4][:, 12:, :8], gp[5][:, 12:], gp[2][:, :, :8], gp[3])
+    assert ok.mean() > 0.5
+    for a, r in zip(ge, ref):
+        assert np.abs(a - r)[ok].max() < 1e-3 * max(1.0, np.abs(r[ok]).max())
