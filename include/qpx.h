@@ -60,7 +60,7 @@ QPX_API size_t qpx_workspace_size(int64_t batch, int32_t n, int32_t m_eq, int32_
 /* qpax.solve_qp -- replaces qpax/pdip.py:181-251 (solve_qp), the driver of initialize
  * (pdip.py:15-35), the predictor-corrector loop (pdip.py:87-119, :228-238), inf-bound removal
  * (pdip.py:166-178, :243-246) and Q symmetrisation (pdip.py:223).  Requires m_ineq >= 1 (the
- * reference's static branches for m_ineq == 0 are qpx_solve_eq_only / qpx_solve_unconstrained).
+ * reference's static branches for m_ineq == 0 are qpx_solve_qp_special).
  * Outputs: x (B,n), s (B,m_ineq), z (B,m_ineq), y (B,m_eq), converged (B), iters (B). */
 QPX_API qpx_status qpx_solve_qp(int32_t dtype, int32_t mode, int64_t batch, int32_t n,
                                 int32_t m_eq, int32_t m_ineq, const void* Q, const void* q,
@@ -95,6 +95,16 @@ QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32
                                const void* dl_dx, int32_t flags, void* dQ, void* dq, void* dA,
                                void* db, void* dG, void* dh, void* workspace,
                                size_t workspace_bytes, void* stream);
+
+/* The static shape branches of qpax.solve_qp for m_ineq == 0 (qpax/pdip.py:208-217): with
+ * m_eq == 0, x = solve(Q, -q) (pdip.py:210); with m_eq > 0, solve_eq_only (pdip.py:137-150).  Both
+ * use the unsymmetrised Q, factored by LU with partial pivoting.  converged = 1, iters = 0.
+ * Workspace: qpx_special_workspace_size(batch, n, m_eq) bytes. */
+QPX_API qpx_status qpx_solve_qp_special(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq,
+                                        const void* Q, const void* q, const void* A, const void* b,
+                                        void* x, void* y, int32_t* converged, int32_t* iters,
+                                        void* workspace, size_t workspace_bytes, void* stream);
+QPX_API size_t qpx_special_workspace_size(int64_t batch, int32_t n, int32_t m_eq);
 
 /* Elastic QP:  minimise 1/2 x'Qx + q'x + penalty 1't  s.t.  G x - t <= h, t >= 0 [, A x = b].
  * qpx_solve_qp_elastic replaces qpax.solve_qp_elastic (qpax/elastic_qp.py:184-228: the

@@ -35,6 +35,8 @@ SIGNATURES = {
                                      + [_vp] * 18 + [_vp, _sz, _vp]),
     "qpx_diff_qp_elastic": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_vp] * 8 + [_i32] + [_vp] * 6
                             + [_vp, _sz, _vp]),
+    "qpx_solve_qp_special": (_i32, [_i32, _i64, _i32, _i32] + [_vp] * 4 + [_vp] * 4 + [_vp, _sz, _vp]),
+    "qpx_special_workspace_size": (_sz, [_i64, _i32, _i32]),
 }
 QPX_EXACT_DG, QPX_ELASTIC_CONSISTENT = 1, 2
 

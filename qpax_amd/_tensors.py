@@ -64,14 +64,14 @@ class Batch:
         return torch.empty((self.B, *tail), dtype=dtype or self.dtype, device=self.device)
 
     def unflatten(self, t, *tail):
-        return t.reshape(*self.batch_shape, *tail)
+        return t.reshape(tuple(self.batch_shape) + tuple(tail))
 
 
-def normalize(Q, q, A, b, G, h):
+def normalize(Q, q, A, b, G, h, device=None):
     """Returns a Batch of (Q, q, A, b, G, h) with shapes (B,n,n) (B,n) (B,me,n) (B,me) (B,mi,n)
-    (B,mi)."""
+    (B,mi).  ``device`` defaults to the inputs' GPU (or the current one)."""
     raw = [as_tensor(v) for v in (Q, q, A, b, G, h)]
-    device = pick_device(*raw)
+    device = device if device is not None else pick_device(*raw)
     dt = result_dtype(*raw)
     Q, q, A, b, G, h = [t.to(device=device, dtype=dt) for t in raw]
     n = q.shape[-1]

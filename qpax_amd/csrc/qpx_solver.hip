@@ -237,6 +237,144 @@ __global__ __launch_bounds__(NT) void k_fused_bwd(Args<TS> a) {
   bwd_solve_write<NT, TS>(a, v, p);
 }
 
+
+// ------------------------------ static special branches --------------------------------------
+// qpax/pdip.py:208-217: with no inequality rows the reference skips the interior-point method.
+// No constraints: x = solve(Q, -q) (jnp.linalg.solve, LU with partial pivoting) on the
+// unsymmetrised Q.  Equality rows only: solve_eq_only (pdip.py:137-150, Boyd p.559),
+// x = Q^{-1}(-A^T y - q) with y = S^{-1}(A Q^{-1} q + b), S = -A Q^{-1} A^T, also on the
+// unsymmetrised Q.  Both factor with partial pivoting (the reference uses LU resp. QR).
+
+// In-place LU with partial pivoting of M (m x m, row-major); perm[i] = original row of row i.
+template <int NT>
+__device__ void block_lu(double* M, int m, int* perm, double* red) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m; i += NT) perm[i] = i;
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {
+    // pivot: first row of maximal |M[i][j]|, i >= j
+    double best = -1.0;
+    int bi = j;
+    for (int i = j + tid; i < m; i += NT) {
+      const double a = fabs(M[i * m + j]);
+      if (a > best) { best = a; bi = i; }
+    }
+    const double bmax = block_reduce<NT>(best, red, OpMax());
+    int cand = (best == bmax) ? bi : m;
+    // lowest row index attaining the maximum
+    for (int o = 32; o > 0; o >>= 1) cand = min(cand, __shfl_xor(cand, o, 64));
+    if constexpr (NT > 64) {
+      __syncthreads();
+      if ((tid & 63) == 0) reinterpret_cast<int*>(red)[tid >> 6] = cand;
+      __syncthreads();
+      for (int w = 0; w < NT / 64; ++w) cand = min(cand, reinterpret_cast<int*>(red)[w]);
+    }
+    const int piv = cand < m ? cand : j;
+    __syncthreads();
+    if (piv != j) {
+      for (int c = tid; c < m; c += NT) {
+        const double t = M[j * m + c];
+        M[j * m + c] = M[piv * m + c];
+        M[piv * m + c] = t;
+      }
+      if (tid == 0) { const int t = perm[j]; perm[j] = perm[piv]; perm[piv] = t; }
+    }
+    __syncthreads();
+    const double inv = 1.0 / M[j * m + j];
+    for (int i = j + 1 + tid; i < m; i += NT) M[i * m + j] *= inv;
+    __syncthreads();
+    const int r = m - j - 1;
+    for (int e = tid; e < r * r; e += NT) {
+      const int a = e / r, c = e - a * r;
+      const int i = j + 1 + a, k = j + 1 + c;
+      M[i * m + k] -= M[i * m + j] * M[j * m + k];
+    }
+    __syncthreads();
+  }
+}
+
+// Solve (LU) X = P B for `nrhs` right-hand sides stored column-major in B (m x nrhs, column c at
+// B + c*m); results overwrite B.  tmp holds m * nrhs doubles.
+template <int NT>
+__device__ void block_lu_solve(const double* LU, const int* perm, int m, double* B, int nrhs, double* tmp) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < m * nrhs; e += NT) {
+    const int c = e / m, i = e - c * m;
+    tmp[e] = B[c * m + perm[i]];
+  }
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {  // unit lower
+    for (int e = tid; e < (m - j - 1) * nrhs; e += NT) {
+      const int c = e / (m - j - 1), i = j + 1 + (e - c * (m - j - 1));
+      tmp[c * m + i] -= LU[i * m + j] * tmp[c * m + j];
+    }
+    __syncthreads();
+  }
+  for (int j = m - 1; j >= 0; --j) {  // upper
+    for (int c = tid; c < nrhs; c += NT) tmp[c * m + j] /= LU[j * m + j];
+    __syncthreads();
+    for (int e = tid; e < j * nrhs; e += NT) {
+      const int c = e / j, i = e - c * j;
+      tmp[c * m + i] -= LU[i * m + j] * tmp[c * m + j];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * nrhs; e += NT) B[e] = tmp[e];
+  __syncthreads();
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_special(Args<TS> a) {
+  __shared__ double red[NT / 64];
+  const int64_t p = blockIdx.x;
+  const int n = a.n, me = a.me, tid = threadIdx.x;
+  const View v = view_of(a, p);
+  // workspace reuse: H (n x n) the LU of Q, G-region for right-hand sides, F the LU of S
+  double* LU = v.H;
+  int* perm = reinterpret_cast<int*>(v.hdiag);
+  int* permS = reinterpret_cast<int*>(v.fdiag);
+  double* rhs = v.G;                 // (me + 1) columns of length n
+  double* tmp = v.G + (int64_t)(me + 1) * n;
+  for (int e = tid; e < n * n; e += NT) LU[e] = (double)a.Q[p * n * n + e];
+  for (int i = tid; i < n; i += NT) rhs[me * n + i] = (double)a.q[p * n + i];
+  for (int e = tid; e < me * n; e += NT) {
+    const int c = e / n, i = e - c * n;
+    rhs[c * n + i] = (double)a.A[(p * me + c) * n + i];  // column c = A^T e_c
+  }
+  __syncthreads();
+  block_lu<NT>(LU, n, perm, red);
+  block_lu_solve<NT>(LU, perm, n, rhs, me + 1, tmp);  // rhs <- [Q^{-1} A^T, Q^{-1} q]
+  const double* Xq = rhs + me * n;
+  if (me == 0) {
+    for (int i = tid; i < n; i += NT) a.x[p * n + i] = (TS)(-Xq[i]);
+  } else {
+    double* S = v.F;
+    double* ys = v.frcp;   // me
+    double* ytmp = v.t;    // me
+    for (int e = tid; e < me * me; e += NT) {
+      const int r = e / me, c = e - r * me;
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) acc += (double)a.A[(p * me + r) * n + i] * rhs[c * n + i];
+      S[e] = -acc;
+    }
+    for (int r = tid; r < me; r += NT) {
+      double acc = (double)a.b[p * me + r];
+      for (int i = 0; i < n; ++i) acc += (double)a.A[(p * me + r) * n + i] * Xq[i];
+      ys[r] = acc;
+    }
+    __syncthreads();
+    block_lu<NT>(S, me, permS, red);
+    block_lu_solve<NT>(S, permS, me, ys, 1, ytmp);
+    for (int i = tid; i < n; i += NT) {
+      double acc = -Xq[i];
+      for (int c = 0; c < me; ++c) acc -= rhs[c * n + i] * ys[c];
+      a.x[p * n + i] = (TS)acc;
+    }
+    for (int c = tid; c < me; c += NT) a.y[p * me + c] = (TS)ys[c];
+  }
+  if (tid == 0) { a.conv[p] = 1; a.iters[p] = 0; }
+}
+
 }  // namespace qpx
 
 // ===================================== host side =============================================
@@ -609,6 +747,41 @@ QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32
                                 db, dG, dh, workspace, stream);
   return backward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, xr, sr, zr, yr, dl_dx, flags, dQ, dq, dA,
                                db, dG, dh, workspace, stream);
+}
+
+QPX_API qpx_status qpx_solve_qp_special(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, const void* Q,
+                                        const void* q, const void* A, const void* b, void* x, void* y,
+                                        int32_t* converged, int32_t* iters, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  g_err.clear();
+  if (dtype != QPX_F32 && dtype != QPX_F64) return fail(QPX_ERR_ARG, "dtype must be QPX_F32 or QPX_F64 (got %d)", dtype);
+  if (batch < 0 || n < 1 || m_eq < 0) return fail(QPX_ERR_ARG, "invalid sizes batch=%lld n=%d m_eq=%d", (long long)batch, n, m_eq);
+  if (batch > 0x7fffffffLL) return fail(QPX_ERR_UNSUPPORTED, "batch %lld exceeds 2^31-1", (long long)batch);
+  // the G region of the workspace view holds 2 (m_eq + 1) columns of length n: sized by m_ineq
+  const int mi = 2 * (m_eq + 1);
+  const size_t need = ws_bytes(batch, n, m_eq, mi);
+  if (workspace_bytes < need) return fail(QPX_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+  if (batch > 0 && any_null({Q, q, x, converged, iters, workspace})) return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (batch == 0) return QPX_OK;
+  auto launch = [&](auto tag) -> qpx_status {
+    using TS = decltype(tag);
+    auto a = make_args<TS>(batch, n, m_eq, mi, workspace);
+    a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+    a.x = (TS*)x; a.y = (TS*)y; a.conv = converged; a.iters = iters;
+    return dispatch_nt(n, 1, [&](auto nt) {
+      hipLaunchKernelGGL((qpx::k_special<decltype(nt)::value, TS>), dim3((unsigned)batch), dim3(decltype(nt)::value), 0,
+                         (hipStream_t)stream, a);
+      QPX_HIP(hipGetLastError());
+      return QPX_OK;
+    });
+  };
+  return dtype == QPX_F32 ? launch(float()) : launch(double());
+}
+
+QPX_API size_t qpx_special_workspace_size(int64_t batch, int32_t n, int32_t m_eq) {
+  if (batch < 0 || n < 1 || m_eq < 0) return 0;
+  return ws_bytes(batch, n, m_eq, 2 * (m_eq + 1));
 }
 
 QPX_API qpx_status qpx_solve_qp_elastic(int32_t dtype, int32_t mode, int64_t batch, int32_t n, int32_t m_eq,
