@@ -50,6 +50,19 @@ def bytes_per_qp(n, me, mi, isz=4):
     return inputs * isz, fwd_out, (n + inputs) * isz
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary (FETCH_SIZE x2 +
+    WRITE_SIZE, calibrated on a plain copy), or None."""
+    path = os.path.join(HERE, "profiles", "k_wave_solve_c3_fp32.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def make_inputs(torch, dev):
     from qpax_amd import problems as P
 
@@ -198,13 +211,15 @@ def main():
         "iters_relax_mean": float(it_r.mean()),
         "kernel_ms": {"fused_solve_relax": round(f_ms, 4), "fused_backward": round(b_ms, 4)},
         "roofline": {
-            "kernel": "qpx::k_fused_solve (solve + relax, float64 VALU)",
+            "kernel": "qpx::wave::k_wave_solve<float,64,2> (solve + relax; float64 MFMA + VALU)",
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": PEAK_F64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_F64_TFLOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("k_wave_solve"),
+            "traffic_source": "profiles/k_wave_solve_c3_fp32.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per launch)",
+            "algorithmic_flops_per_launch": fl_fwd,
         },
     }
     if not args.no_cpu:
