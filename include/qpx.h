@@ -146,6 +146,65 @@ QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batc
                                        void* dG, void* dh, void* workspace, size_t workspace_bytes,
                                        void* stream);
 
+/* ---------------------------------- ragged batches ---------------------------------------------
+ * Problems of different sizes in one call.  No counterpart in the reference, which batches one
+ * size with jax.vmap over solve_qp / solve_qp_primal (qpax/pdip.py:181, qpax/diff_qp.py:50-79);
+ * per problem the arithmetic is exactly that of qpx_solve_qp_relaxed / qpx_diff_qp in
+ * QPX_FUSED mode.
+ *
+ * Operands are packed back to back in problem order: Q_0 (n_0 x n_0) then Q_1 ..., and likewise
+ * q, A (m_eq_i x n_i), b, G (m_ineq_i x n_i), h.  qpx_plan_offsets gives each problem's element
+ * offsets; outputs reuse them (x at q's offset, s and z at h's, y at b's, dQ at Q's, ...).
+ * A plan is bound to the device current at creation and holds device tables for the size pattern;
+ * calls through it allocate nothing and do not synchronise (capturable in a hipGraph).  Calls that
+ * share one workspace must be stream-ordered. */
+typedef struct qpx_plan qpx_plan;
+
+/* Buckets the problems by kernel variant, orders each bucket longest-first and uploads the tables
+ * (synchronous).  Every problem needs n >= 1 and m_ineq >= 1. */
+QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* n, const int32_t* m_eq,
+                                   const int32_t* m_ineq, qpx_plan** plan);
+QPX_API void qpx_plan_destroy(qpx_plan* plan);
+/* info[0..5]: total elements of packed Q, q, A, b, G, h; info[6]: workspace bytes; info[7]: buckets. */
+QPX_API qpx_status qpx_plan_info(const qpx_plan* plan, int64_t* info);
+QPX_API size_t qpx_plan_workspace_size(const qpx_plan* plan);
+/* offsets (batch x 6, host): element offsets of problem i in packed Q, q, A, b, G, h. */
+QPX_API qpx_status qpx_plan_offsets(const qpx_plan* plan, int64_t* offsets);
+/* out[0..9] of bucket i: kind (0 wavefront kernel, 1 workgroup kernel), NMAX, MIC, threads,
+ * wavefronts per workgroup, persistent, problems, LDS bytes per wavefront, workgroups, flops per
+ * iteration. */
+QPX_API qpx_status qpx_plan_bucket(const qpx_plan* plan, int32_t i, int64_t* out);
+/* qpx_solve_qp over a ragged batch (packed operands and outputs). */
+QPX_API qpx_status qpx_plan_solve(const qpx_plan* plan, const void* Q, const void* q, const void* A,
+                                  const void* b, const void* G, const void* h, double solver_tol,
+                                  void* x, void* s, void* z, void* y, int32_t* converged,
+                                  int32_t* iters, void* workspace, size_t workspace_bytes,
+                                  void* stream);
+/* qpx_solve_qp_relaxed over a ragged batch. */
+QPX_API qpx_status qpx_plan_solve_relaxed(const qpx_plan* plan, const void* Q, const void* q,
+                                          const void* A, const void* b, const void* G,
+                                          const void* h, double solver_tol, double target_kappa,
+                                          void* x, void* s, void* z, void* y, int32_t* converged,
+                                          int32_t* iters, void* xr, void* sr, void* zr, void* yr,
+                                          int32_t* relax_converged, int32_t* relax_iters,
+                                          void* workspace, size_t workspace_bytes, void* stream);
+/* qpx_diff_qp over a ragged batch. */
+QPX_API qpx_status qpx_plan_diff(const qpx_plan* plan, const void* Q, const void* q, const void* A,
+                                 const void* b, const void* G, const void* h, const void* xr,
+                                 const void* sr, const void* zr, const void* yr, const void* dl_dx,
+                                 int32_t flags, void* dQ, void* dq, void* dA, void* db, void* dG,
+                                 void* dh, void* workspace, size_t workspace_bytes, void* stream);
+/* Autotuner: times each bucket's candidate launch shapes (wavefronts per workgroup 1/2/4 or
+ * threads 64/128/256, persistent queue or one workgroup per problem) with a solve + relax on
+ * these operands, keeps the fastest in the plan, then runs the call once more with the choice.
+ * Synchronises `stream`.  Same arguments as qpx_plan_solve_relaxed. */
+QPX_API qpx_status qpx_plan_tune(qpx_plan* plan, const void* Q, const void* q, const void* A,
+                                 const void* b, const void* G, const void* h, double solver_tol,
+                                 double target_kappa, void* x, void* s, void* z, void* y,
+                                 int32_t* converged, int32_t* iters, void* xr, void* sr, void* zr,
+                                 void* yr, int32_t* relax_converged, int32_t* relax_iters,
+                                 void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

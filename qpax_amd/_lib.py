@@ -37,6 +37,16 @@ SIGNATURES = {
                             + [_vp, _sz, _vp]),
     "qpx_solve_qp_special": (_i32, [_i32, _i64, _i32, _i32] + [_vp] * 4 + [_vp] * 4 + [_vp, _sz, _vp]),
     "qpx_special_workspace_size": (_sz, [_i64, _i32, _i32]),
+    "qpx_plan_create": (_i32, [_i32, _i64, _vp, _vp, _vp, _vp]),
+    "qpx_plan_destroy": (None, [_vp]),
+    "qpx_plan_info": (_i32, [_vp, _vp]),
+    "qpx_plan_workspace_size": (_sz, [_vp]),
+    "qpx_plan_offsets": (_i32, [_vp, _vp]),
+    "qpx_plan_bucket": (_i32, [_vp, _i32, _vp]),
+    "qpx_plan_solve": (_i32, [_vp] + [_vp] * 6 + [_f64] + [_vp] * 6 + [_vp, _sz, _vp]),
+    "qpx_plan_solve_relaxed": (_i32, [_vp] + [_vp] * 6 + [_f64, _f64] + [_vp] * 12 + [_vp, _sz, _vp]),
+    "qpx_plan_diff": (_i32, [_vp] + [_vp] * 11 + [_i32] + [_vp] * 6 + [_vp, _sz, _vp]),
+    "qpx_plan_tune": (_i32, [_vp] + [_vp] * 6 + [_f64, _f64] + [_vp] * 12 + [_vp, _sz, _vp]),
 }
 QPX_EXACT_DG, QPX_ELASTIC_CONSISTENT = 1, 2
 

@@ -40,7 +40,8 @@ WAVE_INSTANCES = [(ts, nmax, mic) for ts in ("float", "double")
 
 def _units():
     inc = ["-I", os.path.join(HERE, "..", "include")]
-    units = [(os.path.join(CSRC, "qpx_solver.hip"), os.path.join(OBJ, "qpx_solver.o"), inc)]
+    units = [(os.path.join(CSRC, "qpx_solver.hip"), os.path.join(OBJ, "qpx_solver.o"), inc),
+             (os.path.join(CSRC, "qpx_ragged.hip"), os.path.join(OBJ, "qpx_ragged.o"), inc)]
     for ts, nmax, mic in WAVE_INSTANCES:
         defs = [f"-DQPX_TS={ts}", f"-DQPX_NMAX={nmax}", f"-DQPX_MIC={mic}"]
         units.append((os.path.join(CSRC, "qpx_wave_inst.hip"),

@@ -4,6 +4,14 @@
 
 namespace qpx {
 
+// One problem of a ragged batch: its sizes and the element offsets of its slices in the packed
+// operand buffers (outputs reuse them: x and dq at off_q, s, z and dh at off_h, y and db at off_b,
+// dQ at off_Q, dA at off_A, dG at off_G) and the float64-word offset of its workspace slice.
+struct RaggedEntry {
+  int32_t n, me, mi, reserved;
+  int64_t off_Q, off_q, off_A, off_b, off_G, off_h, off_ws;
+};
+
 template <typename TS>
 struct Args {
   int64_t B;
@@ -24,6 +32,28 @@ struct Args {
   const TS* pen;
   TS *t, *s2, *z2, *tr, *s2r, *z2r;
   const TS *tr_in, *s2r_in, *z2r_in;
+  // ragged batches and persistent scheduling
+  const RaggedEntry* rg;  // per-problem sizes and offsets (null: uniform batch of size n, me, mi)
+  const int* order;       // problems of this launch, in scheduling order (null: 0..B-1)
+  int order_count;
+  unsigned* qhead;        // work-queue head for persistent launches (null: one problem per slot)
+  int lds_per_wave;       // bytes of LDS per wavefront (wavefront-per-QP kernels)
 };
+
+// Sizes and slice offsets of problem p.
+struct PIdx {
+  int n, me, mi;
+  int64_t oQ, oq, oA, ob, oG, oh, ows;
+};
+
+template <typename TS>
+__host__ __device__ inline PIdx pidx(const Args<TS>& a, int64_t p) {
+  if (a.rg) {
+    const RaggedEntry& e = a.rg[p];
+    return PIdx{e.n, e.me, e.mi, e.off_Q, e.off_q, e.off_A, e.off_b, e.off_G, e.off_h, e.off_ws};
+  }
+  const int64_t n = a.n, me = a.me, mi = a.mi;
+  return PIdx{a.n, a.me, a.mi, p * n * n, p * n, p * me * n, p * me, p * mi * n, p * mi, p * a.stride};
+}
 
 }  // namespace qpx

@@ -16,6 +16,7 @@
 
 #include "../../include/qpx.h"
 #include "qpx_args.hpp"
+#include "qpx_host.hpp"
 #include "qpx_core.hpp"
 #include "qpx_elastic.hpp"
 
@@ -28,8 +29,32 @@ namespace qpx {
 namespace qpx {
 
 template <typename TS>
+__device__ __forceinline__ View view_of(const Args<TS>& a, const PIdx& pi) {
+  return carve(a.ws + pi.ows, pi.n, pi.me, pi.mi);
+}
+
+template <typename TS>
 __device__ __forceinline__ View view_of(const Args<TS>& a, int64_t p) {
-  return carve(a.ws + p * a.stride, a.n, a.me, a.mi);
+  return view_of(a, pidx(a, p));
+}
+
+// Next problem for the calling workgroup: entry blockIdx.x of the schedule once, or, with a work
+// queue, the entry whose index thread 0 takes from the queue head.
+template <typename TS>
+__device__ int64_t next_problem_block(const Args<TS>& a, int& round, int* slot) {
+  int64_t idx;
+  if (a.qhead) {
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = (int)atomicAdd(a.qhead, 1u);
+    __syncthreads();
+    idx = *slot;
+  } else {
+    if (round++ > 0) return -1;
+    idx = blockIdx.x;
+  }
+  const int64_t count = a.order ? a.order_count : a.B;
+  if (idx >= count) return -1;
+  return a.order ? (int64_t)a.order[idx] : idx;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -38,11 +63,9 @@ __device__ __forceinline__ View view_of(const Args<TS>& a, int64_t p) {
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_setup(Args<TS> a) {
   __shared__ double red[NT / 64];
-  const int64_t p = blockIdx.x;
-  const View v = view_of(a, p);
-  const int64_t n = a.n, me = a.me, mi = a.mi;
-  stage_load<NT, TS>(v, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me,
-                     a.G + p * mi * n, a.h + p * mi);
+  const PIdx pi = pidx(a, blockIdx.x);
+  const View v = view_of(a, pi);
+  stage_load<NT, TS>(v, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
   stage_init<NT>(v, red);
 }
 
@@ -105,14 +128,15 @@ __global__ __launch_bounds__(NT) void k_step(Args<TS> a, int relax) {
 // Post-processing of solve_qp (pdip.py:243-251) and start of the relaxation.
 template <int NT, typename TS>
 __device__ void finish_solve(const Args<TS>& a, const View& v, int64_t p) {
-  const int n = a.n, me = a.me, mi = a.mi, tid = threadIdx.x;
+  const PIdx pi = pidx(a, p);
+  const int n = pi.n, me = pi.me, mi = pi.mi, tid = threadIdx.x;
   for (int k = tid; k < mi; k += NT) {
     if (v.hinf[k] != 0.0) { v.z[k] = 0.0; v.s[k] = INFINITY; }
-    a.s[p * mi + k] = (TS)v.s[k];
-    a.z[p * mi + k] = (TS)v.z[k];
+    a.s[pi.oh + k] = (TS)v.s[k];
+    a.z[pi.oh + k] = (TS)v.z[k];
   }
-  for (int i = tid; i < n; i += NT) a.x[p * n + i] = (TS)v.x[i];
-  for (int c = tid; c < me; c += NT) a.y[p * me + c] = (TS)v.y[c];
+  for (int i = tid; i < n; i += NT) a.x[pi.oq + i] = (TS)v.x[i];
+  for (int c = tid; c < me; c += NT) a.y[pi.ob + c] = (TS)v.y[c];
   __syncthreads();
   if (tid == 0) {
     a.conv[p] = v.ctl[kConv];
@@ -125,10 +149,11 @@ __device__ void finish_solve(const Args<TS>& a, const View& v, int64_t p) {
 
 template <int NT, typename TS>
 __device__ void finish_relax(const Args<TS>& a, const View& v, int64_t p) {
-  const int n = a.n, me = a.me, mi = a.mi, tid = threadIdx.x;
-  for (int k = tid; k < mi; k += NT) { a.sr[p * mi + k] = (TS)v.s[k]; a.zr[p * mi + k] = (TS)v.z[k]; }
-  for (int i = tid; i < n; i += NT) a.xr[p * n + i] = (TS)v.x[i];
-  for (int c = tid; c < me; c += NT) a.yr[p * me + c] = (TS)v.y[c];
+  const PIdx pi = pidx(a, p);
+  const int n = pi.n, me = pi.me, mi = pi.mi, tid = threadIdx.x;
+  for (int k = tid; k < mi; k += NT) { a.sr[pi.oh + k] = (TS)v.s[k]; a.zr[pi.oh + k] = (TS)v.z[k]; }
+  for (int i = tid; i < n; i += NT) a.xr[pi.oq + i] = (TS)v.x[i];
+  for (int c = tid; c < me; c += NT) a.yr[pi.ob + c] = (TS)v.y[c];
   if (tid == 0) { a.rconv[p] = v.ctl[kRelaxConv]; a.riters[p] = v.ctl[kRelaxIter]; }
 }
 
@@ -146,70 +171,73 @@ __global__ __launch_bounds__(NT) void k_finish_relax(Args<TS> a) {
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_fused_solve(Args<TS> a, int with_relax) {
   __shared__ double red[NT / 64];
-  const int64_t p = blockIdx.x;
-  const View v = view_of(a, p);
-  const int64_t n = a.n, me = a.me, mi = a.mi;
-  stage_load<NT, TS>(v, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me,
-                     a.G + p * mi * n, a.h + p * mi);
-  stage_init<NT>(v, red);
-  while (is_active(v, false)) {
-    __syncthreads();
-    pc_iteration<NT>(v, a.tol, red);
+  __shared__ int slot;
+  int round = 0;
+  for (int64_t p; (p = next_problem_block(a, round, &slot)) >= 0;) {
+    const PIdx pi = pidx(a, p);
+    const View v = view_of(a, pi);
+    stage_load<NT, TS>(v, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
+    stage_init<NT>(v, red);
+    while (is_active(v, false)) {
+      __syncthreads();
+      pc_iteration<NT>(v, a.tol, red);
+    }
+    finish_solve<NT, TS>(a, v, p);
+    if (!with_relax) continue;
+    while (is_active(v, true)) {
+      __syncthreads();
+      relax_iteration<NT>(v, a.tol, a.kappa, red);
+    }
+    finish_relax<NT, TS>(a, v, p);
   }
-  finish_solve<NT, TS>(a, v, p);
-  if (!with_relax) return;
-  while (is_active(v, true)) {
-    __syncthreads();
-    relax_iteration<NT>(v, a.tol, a.kappa, red);
-  }
-  finish_relax<NT, TS>(a, v, p);
 }
 
 // ------------------------------------ backward ---------------------------------------------
 // diff_qp (qpax/diff_qp.py:20-46): load the relaxed point and the cotangent.
 template <int NT, typename TS>
 __device__ void bwd_load(const Args<TS>& a, const View& v, int64_t p) {
-  const int64_t n = a.n, me = a.me, mi = a.mi;
-  stage_load<NT, TS>(v, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me,
-                     a.G + p * mi * n, a.h + p * mi);
+  const PIdx pi = pidx(a, p);
+  const int n = pi.n, me = pi.me, mi = pi.mi;
+  stage_load<NT, TS>(v, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += NT) { v.x[i] = (double)a.xr_in[p * n + i]; v.r1[i] = (double)a.g[p * n + i]; }
+  for (int i = tid; i < n; i += NT) { v.x[i] = (double)a.xr_in[pi.oq + i]; v.r1[i] = (double)a.g[pi.oq + i]; }
   for (int k = tid; k < mi; k += NT) {
-    v.s[k] = (double)a.sr_in[p * mi + k];
-    v.z[k] = (double)a.zr_in[p * mi + k];
+    v.s[k] = (double)a.sr_in[pi.oh + k];
+    v.z[k] = (double)a.zr_in[pi.oh + k];
     v.r2[k] = 0.0;
     v.r3[k] = 0.0;
   }
-  for (int c = tid; c < me; c += NT) { v.y[c] = (double)a.yr_in[p * me + c]; v.r4[c] = 0.0; }
+  for (int c = tid; c < me; c += NT) { v.y[c] = (double)a.yr_in[pi.ob + c]; v.r4[c] = 0.0; }
   __syncthreads();
 }
 
 // Solve with rhs (-dl_dx, 0, 0, 0) and form the cotangents (optnet_derivatives, diff_qp.py:8-17).
 template <int NT, typename TS>
 __device__ void bwd_solve_write(const Args<TS>& a, const View& v, int64_t p) {
-  const int n = a.n, me = a.me, mi = a.mi, tid = threadIdx.x;
+  const PIdx pi = pidx(a, p);
+  const int n = pi.n, me = pi.me, mi = pi.mi, tid = threadIdx.x;
   stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
   // dz here is dlam_tilde; dlam = dlam_tilde / lam
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e - i * n;
-    a.dQ[p * n * n + e] = (TS)(0.5 * (v.dx[i] * v.x[j] + v.x[i] * v.dx[j]));
+    a.dQ[pi.oQ + e] = (TS)(0.5 * (v.dx[i] * v.x[j] + v.x[i] * v.dx[j]));
   }
-  for (int i = tid; i < n; i += NT) a.dq[p * n + i] = (TS)v.dx[i];
+  for (int i = tid; i < n; i += NT) a.dq[pi.oq + i] = (TS)v.dx[i];
   for (int e = tid; e < me * n; e += NT) {
     const int c = e / n, j = e - c * n;
-    a.dA[(int64_t)p * me * n + e] = (TS)(v.dy[c] * v.x[j] + v.y[c] * v.dx[j]);
+    a.dA[pi.oA + e] = (TS)(v.dy[c] * v.x[j] + v.y[c] * v.dx[j]);
   }
-  for (int c = tid; c < me; c += NT) a.db[p * me + c] = (TS)(-v.dy[c]);
+  for (int c = tid; c < me; c += NT) a.db[pi.ob + c] = (TS)(-v.dy[c]);
   for (int e = tid; e < mi * n; e += NT) {
     const int k = e / n, j = e - k * n;
     const double lam = v.z[k], dlam = v.dz[k] / lam;
     // reference: diag(lam) (dlam x^T + lam dx^T) (diff_qp.py:11); exact VJP: diag(lam) dlam x^T + lam dx^T
     const double val = (a.flags & 1) ? lam * dlam * v.x[j] + lam * v.dx[j] : lam * (dlam * v.x[j] + lam * v.dx[j]);
-    a.dG[(int64_t)p * mi * n + e] = (TS)val;
+    a.dG[pi.oG + e] = (TS)val;
   }
   for (int k = tid; k < mi; k += NT) {
     const double lam = v.z[k], dlam = v.dz[k] / lam;
-    a.dh[p * mi + k] = (TS)(-lam * dlam);
+    a.dh[pi.oh + k] = (TS)(-lam * dlam);
   }
 }
 
@@ -230,11 +258,15 @@ __global__ __launch_bounds__(NT) void k_bwd_solve(Args<TS> a) {
 
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_fused_bwd(Args<TS> a) {
-  const int64_t p = blockIdx.x;
-  const View v = view_of(a, p);
-  bwd_load<NT, TS>(a, v, p);
-  stage_factor<NT>(v, kDzs);
-  bwd_solve_write<NT, TS>(a, v, p);
+  __shared__ int slot;
+  int round = 0;
+  for (int64_t p; (p = next_problem_block(a, round, &slot)) >= 0;) {
+    const View v = view_of(a, p);
+    bwd_load<NT, TS>(a, v, p);
+    stage_factor<NT>(v, kDzs);
+    bwd_solve_write<NT, TS>(a, v, p);
+    __syncthreads();
+  }
 }
 
 
@@ -378,9 +410,47 @@ __global__ __launch_bounds__(NT) void k_special(Args<TS> a) {
 }  // namespace qpx
 
 // ===================================== host side =============================================
-namespace {
+namespace qpx {
 
+// Launchers of the workgroup-per-QP fused kernels, shared with the ragged-batch scheduler.
+template <int NT, typename TS>
+hipError_t launch_generic_solve(const Args<TS>& a, unsigned grid, hipStream_t st, int relax) {
+  hipLaunchKernelGGL((k_fused_solve<NT, TS>), dim3(grid), dim3(NT), 0, st, a, relax);
+  return hipGetLastError();
+}
+
+template <int NT, typename TS>
+hipError_t launch_generic_bwd(const Args<TS>& a, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL((k_fused_bwd<NT, TS>), dim3(grid), dim3(NT), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int NT, typename TS>
+int generic_occupancy() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_fused_solve<NT, TS>), NT, 0) !=
+      hipSuccess)
+    nb = 0;
+  return nb;
+}
+
+#define QPX_GEN_INST(NT, TS)                                                                   \
+  template hipError_t launch_generic_solve<NT, TS>(const Args<TS>&, unsigned, hipStream_t, int); \
+  template hipError_t launch_generic_bwd<NT, TS>(const Args<TS>&, unsigned, hipStream_t);        \
+  template int generic_occupancy<NT, TS>();
+QPX_GEN_INST(64, float)
+QPX_GEN_INST(128, float)
+QPX_GEN_INST(256, float)
+QPX_GEN_INST(64, double)
+QPX_GEN_INST(128, double)
+QPX_GEN_INST(256, double)
+#undef QPX_GEN_INST
+
+namespace host {
+
+namespace {
 thread_local std::string g_err;
+}
 
 qpx_status fail(qpx_status st, const char* fmt, ...) {
   char buf[512];
@@ -392,13 +462,8 @@ qpx_status fail(qpx_status st, const char* fmt, ...) {
   return st;
 }
 
-#define QPX_HIP(call)                                                                  \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess) return fail(QPX_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
-
-constexpr size_t kCtlBytes = 4096;  // head of the workspace: counters
+void clear_error() { g_err.clear(); }
+const char* last_error() { return g_err.c_str(); }
 
 // Per-problem float64 words: the generic kernels' view, or the wavefront kernels' m x m reduced
 // KKT matrix (m = n padded to a multiple of 4, plus m_eq), whichever is larger.
@@ -407,6 +472,46 @@ int64_t stride_words(int n, int me, int mi) {
   const int64_t w = qpx::elastic_words(n, me, mi);
   return w > qpx::round_up(m * m, 64) ? w : qpx::round_up(m * m, 64);
 }
+
+int max_lds_per_block() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0;
+    int val = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) val = 65536;
+    v = val;
+  }
+  return v;
+}
+
+int cu_count() {
+  int dev = 0, val = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&val, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    val = 256;
+  return val;
+}
+
+// The wavefront-per-QP kernels hold the augmented KKT row of a lane in registers, so they take
+// problems with n + m_eq <= 64 and m_ineq <= 256 whose LDS image fits one workgroup.
+bool wave_variant(int n, int me, int mi, int ts, WaveVariant* v) {
+  const int m = qpx::wave::padded_n(n) + me;
+  if (m > 64 || mi > 256) return false;
+  if (qpx::wave::lds_layout(n, me, mi, ts).bytes > max_lds_per_block()) return false;
+  if (m <= 32 && mi <= 64) *v = {32, 1};
+  else if (m <= 32 && mi <= 128) *v = {32, 2};
+  else if (mi <= 64) *v = {64, 1};
+  else if (mi <= 128) *v = {64, 2};
+  else *v = {64, 4};
+  return true;
+}
+
+}  // namespace host
+}  // namespace qpx
+
+namespace {
+using qpx::host::fail;
+using qpx::host::kCtlBytes;
+using qpx::host::stride_words;
 
 size_t ws_bytes(int64_t B, int n, int me, int mi) {
   return kCtlBytes + (size_t)B * (size_t)stride_words(n, me, mi) * sizeof(double);
@@ -443,48 +548,12 @@ int* pinned_counter() {
   return p;
 }
 
-// The wavefront-per-QP kernels hold the augmented KKT row of a lane in registers, so they take
-// problems with n + m_eq <= 64 and m_ineq <= 256 whose LDS image fits one workgroup.
-int max_lds_per_block() {
-  static int v = -1;
-  if (v < 0) {
-    int dev = 0;
-    int val = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) val = 65536;
-    v = val;
-  }
-  return v;
-}
-
+// A uniform batch on the wavefront kernels: one wavefront (workgroup) per problem.
 template <typename TS>
-bool wave_fits(int n, int me, int mi) {
-  if (qpx::wave::padded_n(n) + me > 64 || mi > 256) return false;
-  return qpx::wave::lds_layout(n, me, mi, (int)sizeof(TS)).bytes <= max_lds_per_block();
-}
-
-#define QPX_WAVE_CASES(TS, X)                  \
-  if (m <= 32 && mi <= 64) return X(TS, 32, 1);  \
-  if (m <= 32 && mi <= 128) return X(TS, 32, 2); \
-  if (mi <= 64) return X(TS, 64, 1);             \
-  if (mi <= 128) return X(TS, 64, 2);            \
-  return X(TS, 64, 4);
-
-template <typename TS>
-hipError_t wave_solve(const qpx::Args<TS>& a, int relax, hipStream_t st) {
-  const int m = qpx::wave::padded_n(a.n) + a.me, mi = a.mi;
-  const int lds = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
-#define QPX_X(T, N, M) qpx::wave::launch_wave_solve<T, N, M>(a, (unsigned)a.B, lds, st, relax)
-  QPX_WAVE_CASES(TS, QPX_X)
-#undef QPX_X
-}
-
-template <typename TS>
-hipError_t wave_bwd(const qpx::Args<TS>& a, hipStream_t st) {
-  const int m = qpx::wave::padded_n(a.n) + a.me, mi = a.mi;
-  const int lds = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
-#define QPX_X(T, N, M) qpx::wave::launch_wave_bwd<T, N, M>(a, (unsigned)a.B, lds, st)
-  QPX_WAVE_CASES(TS, QPX_X)
-#undef QPX_X
+bool wave_plan(qpx::Args<TS>& a, qpx::host::WaveVariant* v) {
+  if (!qpx::host::wave_variant(a.n, a.me, a.mi, (int)sizeof(TS), v)) return false;
+  a.lds_per_wave = qpx::wave::lds_layout(a.n, a.me, a.mi, (int)sizeof(TS)).bytes;
+  return true;
 }
 
 template <int NT, typename TS>
@@ -492,8 +561,9 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
-    if (wave_fits<TS>(a.n, a.me, a.mi)) {
-      QPX_HIP(wave_solve<TS>(a, relax ? 1 : 0, st));
+    qpx::host::WaveVariant wv;
+    if (wave_plan(a, &wv)) {
+      QPX_HIP(qpx::wave::launch_solve(wv, a, (unsigned)a.B, 1, st, relax ? 1 : 0));
       return QPX_OK;
     }
     hipLaunchKernelGGL((qpx::k_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
@@ -532,8 +602,9 @@ qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
-    if (wave_fits<TS>(a.n, a.me, a.mi)) {
-      QPX_HIP(wave_bwd<TS>(a, st));
+    qpx::host::WaveVariant wv;
+    if (wave_plan(a, &wv)) {
+      QPX_HIP(qpx::wave::launch_bwd(wv, a, (unsigned)a.B, 1, st));
       return QPX_OK;
     }
     hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, 0, st, a);
@@ -683,7 +754,7 @@ extern "C" {
 
 QPX_API int32_t qpx_version(void) { return 100; }
 
-QPX_API const char* qpx_last_error(void) { return g_err.c_str(); }
+QPX_API const char* qpx_last_error(void) { return qpx::host::last_error(); }
 
 QPX_API size_t qpx_workspace_size(int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq) {
   if (batch < 0 || n < 1 || m_eq < 0 || m_ineq < 0) return 0;
@@ -696,7 +767,7 @@ QPX_API qpx_status qpx_solve_qp(int32_t dtype, int32_t mode, int64_t batch, int3
                                 void* x, void* s, void* z, void* y, int32_t* converged,
                                 int32_t* iters, void* workspace, size_t workspace_bytes,
                                 void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
   if (st != QPX_OK) return st;
   if (batch > 0 && any_null({Q, q, G, h, x, s, z, converged, iters, workspace}))
@@ -717,7 +788,7 @@ QPX_API qpx_status qpx_solve_qp_relaxed(int32_t dtype, int32_t mode, int64_t bat
                                         void* xr, void* sr, void* zr, void* yr,
                                         int32_t* relax_converged, int32_t* relax_iters,
                                         void* workspace, size_t workspace_bytes, void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
   if (st != QPX_OK) return st;
   if (batch > 0 && any_null({Q, q, G, h, x, s, z, converged, iters, xr, sr, zr, relax_converged, relax_iters, workspace}))
@@ -736,7 +807,7 @@ QPX_API qpx_status qpx_diff_qp(int32_t dtype, int32_t mode, int64_t batch, int32
                                const void* sr, const void* zr, const void* yr, const void* dl_dx,
                                int32_t flags, void* dQ, void* dq, void* dA, void* db, void* dG, void* dh,
                                void* workspace, size_t workspace_bytes, void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
   if (st != QPX_OK) return st;
   if (batch > 0 && any_null({Q, q, G, h, xr, sr, zr, dl_dx, dQ, dq, dG, dh, workspace}))
@@ -753,7 +824,7 @@ QPX_API qpx_status qpx_solve_qp_special(int32_t dtype, int64_t batch, int32_t n,
                                         const void* q, const void* A, const void* b, void* x, void* y,
                                         int32_t* converged, int32_t* iters, void* workspace,
                                         size_t workspace_bytes, void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   if (dtype != QPX_F32 && dtype != QPX_F64) return fail(QPX_ERR_ARG, "dtype must be QPX_F32 or QPX_F64 (got %d)", dtype);
   if (batch < 0 || n < 1 || m_eq < 0) return fail(QPX_ERR_ARG, "invalid sizes batch=%lld n=%d m_eq=%d", (long long)batch, n, m_eq);
   if (batch > 0x7fffffffLL) return fail(QPX_ERR_UNSUPPORTED, "batch %lld exceeds 2^31-1", (long long)batch);
@@ -790,7 +861,7 @@ QPX_API qpx_status qpx_solve_qp_elastic(int32_t dtype, int32_t mode, int64_t bat
                                         double solver_tol, void* x, void* t, void* s1, void* s2, void* z1,
                                         void* z2, void* y, int32_t* converged, int32_t* iters, void* workspace,
                                         size_t workspace_bytes, void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
   if (st != QPX_OK) return st;
   if (batch > 0 && any_null({Q, q, G, h, penalty, x, t, s1, s2, z1, z2, converged, iters, workspace}))
@@ -813,7 +884,7 @@ QPX_API qpx_status qpx_solve_qp_elastic_relaxed(int32_t dtype, int32_t mode, int
                                                 void* s2r, void* z1r, void* z2r, void* yr, int32_t* relax_converged,
                                                 int32_t* relax_iters, void* workspace, size_t workspace_bytes,
                                                 void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
   if (st != QPX_OK) return st;
   if (batch > 0 && any_null({Q, q, G, h, penalty, x, t, s1, s2, z1, z2, converged, iters, xr, tr, s1r, s2r, z1r, z2r,
@@ -838,7 +909,7 @@ QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batc
                                        const void* z2r, const void* yr, const void* dl_dx, int32_t flags, void* dQ,
                                        void* dq, void* dA, void* db, void* dG, void* dh, void* workspace,
                                        size_t workspace_bytes, void* stream) {
-  g_err.clear();
+  qpx::host::clear_error();
   qpx_status st = check_dims(dtype, mode, batch, n, m_eq, m_ineq, workspace_bytes);
   if (st != QPX_OK) return st;
   if (batch > 0 && any_null({Q, q, G, h, penalty, xr, tr, s1r, s2r, z1r, z2r, dl_dx, dQ, dq, dG, dh, workspace}))

@@ -529,15 +529,18 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
   return nrm < tol ? 1 : 0;
 }
 
+// Context of one problem for the calling wavefront; several wavefronts of a workgroup each own
+// a disjoint LDS image of lds_per_wave bytes.
 template <typename TS, int NMAX, int MIC>
-__device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, double* ws) {
+__device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, double* ws, int lds_per_wave) {
   extern __shared__ __align__(16) unsigned char qpx_smem[];
   const Lds w = lds_layout(n, me, mi, (int)sizeof(TS));
+  unsigned char* base = qpx_smem + (threadIdx.x >> 6) * lds_per_wave;
   Ctx<TS, NMAX, MIC> c;
-  c.n = n; c.np = padded_n(n); c.me = me; c.mi = mi; c.m = c.np + me; c.l = threadIdx.x;
-  c.Gs = reinterpret_cast<const TS*>(qpx_smem + w.off_G);
-  c.Lp = reinterpret_cast<double*>(qpx_smem + w.off_L);
-  c.pb = reinterpret_cast<double*>(qpx_smem + w.off_P);
+  c.n = n; c.np = padded_n(n); c.me = me; c.mi = mi; c.m = c.np + me; c.l = threadIdx.x & 63;
+  c.Gs = reinterpret_cast<const TS*>(base + w.off_G);
+  c.Lp = reinterpret_cast<double*>(base + w.off_L);
+  c.pb = reinterpret_cast<double*>(base + w.off_P);
   c.col = c.pb;
   c.vb = c.pb + 128;
   c.KQ = ws;
@@ -545,113 +548,138 @@ __device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, do
 }
 
 template <typename TS, int NMAX, int MIC>
-__device__ __forceinline__ void store_state(const Ctx<TS, NMAX, MIC>& c, const St<TS, NMAX, MIC>& st, int64_t p,
+__device__ __forceinline__ void store_state(const Ctx<TS, NMAX, MIC>& c, const St<TS, NMAX, MIC>& st, const PIdx& pi,
                                             TS* x, TS* s, TS* z, TS* y, bool post) {
   const int n = c.n, np = c.np, me = c.me, mi = c.mi, l = c.l;
-  if (l < n) x[p * n + l] = (TS)st.xa;
-  else if (l >= np && l < np + me) y[p * me + (l - np)] = (TS)st.xa;
+  if (l < n) x[pi.oq + l] = (TS)st.xa;
+  else if (l >= np && l < np + me) y[pi.ob + (l - np)] = (TS)st.xa;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if (k < mi) {
       double sv = st.s[cc], zv = st.z[cc];
       if (post && st.hinf[cc]) { zv = 0.0; sv = INFINITY; }
-      s[p * mi + k] = (TS)sv;
-      z[p * mi + k] = (TS)zv;
+      s[pi.oh + k] = (TS)sv;
+      z[pi.oh + k] = (TS)zv;
     }
   }
 }
 
-// Forward kernel: solve (and optionally relax) one QP per wavefront.
+// Next problem for the calling wavefront.  Without a work queue wavefront w of workgroup b takes
+// entry b * (waves per workgroup) + w of the schedule once; with one, lane 0 pulls the next entry.
+template <typename TS>
+__device__ __forceinline__ int64_t next_problem(const Args<TS>& a, int& round) {
+  int64_t idx;
+  if (a.qhead) {
+    unsigned v = 0;
+    if ((threadIdx.x & 63) == 0) v = atomicAdd(a.qhead, 1u);
+    idx = (int64_t)__builtin_amdgcn_readfirstlane(v);
+  } else {
+    if (round++ > 0) return -1;
+    idx = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  }
+  const int64_t count = a.order ? a.order_count : a.B;
+  if (idx >= count) return -1;
+  return a.order ? (int64_t)a.order[idx] : idx;
+}
+
+// Forward kernel: solve (and optionally relax) QPs, one per wavefront at a time.
 template <typename TS, int NMAX, int MIC>
-__global__ __launch_bounds__(64) void k_wave_solve(Args<TS> a, int with_relax) {
-  const int64_t p = blockIdx.x;
-  const int n = a.n, me = a.me, mi = a.mi;
-  const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + p * a.stride);
-  St<TS, NMAX, MIC> st;
-  load(c, st, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me, a.G + p * mi * n, a.h + p * mi);
-  init(c, st);
-  int conv = 0, it = 0;
-  while (it < kMaxIters && conv == 0) {
-    conv = pc_iter(c, st, a.tol);
-    ++it;
-  }
-  store_state(c, st, p, a.x, a.s, a.z, a.y, true);
-  if (c.l == 0) { a.conv[p] = conv; a.iters[p] = it; }
-  if (!with_relax) return;
-  // relax from the post-processed point (s = inf, z = 0 on rows with infinite h)
+__global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) {
+  int round = 0;
+  for (int64_t p; (p = next_problem(a, round)) >= 0;) {
+    const PIdx pi = pidx(a, p);
+    const int n = pi.n, me = pi.me, mi = pi.mi;
+    const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + pi.ows, a.lds_per_wave);
+    St<TS, NMAX, MIC> st;
+    load(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
+    init(c, st);
+    int conv = 0, it = 0;
+    while (it < kMaxIters && conv == 0) {
+      conv = pc_iter(c, st, a.tol);
+      ++it;
+    }
+    store_state(c, st, pi, a.x, a.s, a.z, a.y, true);
+    if (c.l == 0) { a.conv[p] = conv; a.iters[p] = it; }
+    if (!with_relax) continue;
+    // relax from the post-processed point (s = inf, z = 0 on rows with infinite h)
 #pragma unroll
-  for (int cc = 0; cc < MIC; ++cc)
-    if (st.hinf[cc]) { st.z[cc] = 0.0; st.s[cc] = INFINITY; }
-  int rconv = 0, rit = 0;
-  while (rit < kMaxIters && rconv == 0) {
-    rconv = relax_iter(c, st, a.tol, a.kappa);
-    ++rit;
+    for (int cc = 0; cc < MIC; ++cc)
+      if (st.hinf[cc]) { st.z[cc] = 0.0; st.s[cc] = INFINITY; }
+    int rconv = 0, rit = 0;
+    while (rit < kMaxIters && rconv == 0) {
+      rconv = relax_iter(c, st, a.tol, a.kappa);
+      ++rit;
+    }
+    store_state(c, st, pi, a.xr, a.sr, a.zr, a.yr, false);
+    if (c.l == 0) { a.rconv[p] = rconv; a.riters[p] = rit; }
+    WSYNC();
   }
-  store_state(c, st, p, a.xr, a.sr, a.zr, a.yr, false);
-  if (c.l == 0) { a.rconv[p] = rconv; a.riters[p] = rit; }
 }
 
 // Backward kernel (qpax/diff_qp.py:20-46 with optnet_derivatives :8-17).
 template <typename TS, int NMAX, int MIC>
-__global__ __launch_bounds__(64) void k_wave_bwd(Args<TS> a) {
-  const int64_t p = blockIdx.x;
-  const int n = a.n, me = a.me, mi = a.mi;
-  const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + p * a.stride);
-  const int l = c.l, np = c.np, m = c.m;
-  St<TS, NMAX, MIC> st;
-  load(c, st, a.Q + p * n * n, a.q + p * n, a.A + p * me * n, a.b + p * me, a.G + p * mi * n, a.h + p * mi);
-  st.xa = l < n ? (double)a.xr_in[p * n + l] : ((l >= np && l < m) ? (double)a.yr_in[p * me + (l - np)] : 0.0);
+__global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
+  int round = 0;
+  for (int64_t p; (p = next_problem(a, round)) >= 0;) {
+    const PIdx pi = pidx(a, p);
+    const int n = pi.n, me = pi.me, mi = pi.mi;
+    const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + pi.ows, a.lds_per_wave);
+    const int l = c.l, np = c.np, m = c.m;
+    St<TS, NMAX, MIC> st;
+    load(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
+    st.xa = l < n ? (double)a.xr_in[pi.oq + l] : ((l >= np && l < m) ? (double)a.yr_in[pi.ob + (l - np)] : 0.0);
 #pragma unroll
-  for (int cc = 0; cc < MIC; ++cc) {
-    const int k = 64 * cc + l;
-    st.s[cc] = k < mi ? (double)a.sr_in[p * mi + k] : 1.0;
-    st.z[cc] = k < mi ? (double)a.zr_in[p * mi + k] : 1.0;
-  }
-  factor(c, st, false);
-  double zero[MIC];
-#pragma unroll
-  for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;
-  const double g = l < n ? (double)a.g[p * n + l] : 0.0;
-  solve(c, st, -g, zero, zero, 1.0, st.ds, st.dz);
-  // cotangents; dz holds dlam_tilde, dlam = dlam_tilde / lam
-  double* lamv = c.vb;        // lam = z
-  double* dlamv = c.vb + mi;  // dlam = dlam_tilde / lam
-  c.col[l] = st.xa;
-  c.col[64 + l] = st.da;
-#pragma unroll
-  for (int cc = 0; cc < MIC; ++cc) {
-    const int k = 64 * cc + l;
-    if (k < mi) {
-      lamv[k] = st.z[cc];
-      dlamv[k] = st.dz[cc] / st.z[cc];
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      st.s[cc] = k < mi ? (double)a.sr_in[pi.oh + k] : 1.0;
+      st.z[cc] = k < mi ? (double)a.zr_in[pi.oh + k] : 1.0;
     }
-  }
-  WSYNC();
-  const double* X = c.col;
-  const double* DX = c.col + 64;
-  for (int i = 0; i < n; ++i) {
-    for (int j = l; j < n; j += 64)
-      a.dQ[(p * n + i) * n + j] = (TS)(0.5 * (DX[i] * X[j] + X[i] * DX[j]));
-  }
-  if (l < n) a.dq[p * n + l] = (TS)st.da;
-  for (int cix = 0; cix < me; ++cix) {
-    const double dnu = DX[np + cix], nu = X[np + cix];
-    for (int j = l; j < n; j += 64) a.dA[(p * me + cix) * n + j] = (TS)(dnu * X[j] + nu * DX[j]);
-  }
-  if (l >= np && l < m) a.db[p * me + (l - np)] = (TS)(-st.da);
-  const bool exact = a.flags & 1;
-  for (int k = 0; k < mi; ++k) {
-    const double lam = lamv[k], dlam = dlamv[k];
-    for (int j = l; j < n; j += 64) {
-      const double v = exact ? lam * dlam * X[j] + lam * DX[j] : lam * (dlam * X[j] + lam * DX[j]);
-      a.dG[(p * mi + k) * n + j] = (TS)v;
-    }
-  }
+    factor(c, st, false);
+    double zero[MIC];
 #pragma unroll
-  for (int cc = 0; cc < MIC; ++cc) {
-    const int k = 64 * cc + l;
-    if (k < mi) a.dh[p * mi + k] = (TS)(-st.z[cc] * (st.dz[cc] / st.z[cc]));
+    for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;
+    const double g = l < n ? (double)a.g[pi.oq + l] : 0.0;
+    solve(c, st, -g, zero, zero, 1.0, st.ds, st.dz);
+    // cotangents; dz holds dlam_tilde, dlam = dlam_tilde / lam
+    double* lamv = c.vb;        // lam = z
+    double* dlamv = c.vb + mi;  // dlam = dlam_tilde / lam
+    c.col[l] = st.xa;
+    c.col[64 + l] = st.da;
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      if (k < mi) {
+        lamv[k] = st.z[cc];
+        dlamv[k] = st.dz[cc] / st.z[cc];
+      }
+    }
+    WSYNC();
+    const double* X = c.col;
+    const double* DX = c.col + 64;
+    for (int i = 0; i < n; ++i) {
+      for (int j = l; j < n; j += 64) a.dQ[pi.oQ + i * n + j] = (TS)(0.5 * (DX[i] * X[j] + X[i] * DX[j]));
+    }
+    if (l < n) a.dq[pi.oq + l] = (TS)st.da;
+    for (int cix = 0; cix < me; ++cix) {
+      const double dnu = DX[np + cix], nu = X[np + cix];
+      for (int j = l; j < n; j += 64) a.dA[pi.oA + cix * n + j] = (TS)(dnu * X[j] + nu * DX[j]);
+    }
+    if (l >= np && l < m) a.db[pi.ob + (l - np)] = (TS)(-st.da);
+    const bool exact = a.flags & 1;
+    for (int k = 0; k < mi; ++k) {
+      const double lam = lamv[k], dlam = dlamv[k];
+      for (int j = l; j < n; j += 64) {
+        const double v = exact ? lam * dlam * X[j] + lam * DX[j] : lam * (dlam * X[j] + lam * DX[j]);
+        a.dG[pi.oG + k * n + j] = (TS)v;
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      if (k < mi) a.dh[pi.oh + k] = (TS)(-st.z[cc] * (st.dz[cc] / st.z[cc]));
+    }
+    WSYNC();
   }
 }
 

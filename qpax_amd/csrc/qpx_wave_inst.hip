@@ -1,23 +1,36 @@
 // One instantiation of the wavefront-per-QP kernels, selected at compile time with
 // -DQPX_TS=<float|double> -DQPX_NMAX=<32|64> -DQPX_MIC=<1|2|4>.
+#include "qpx_host.hpp"
 #include "qpx_wave.hpp"
 
 namespace qpx {
 namespace wave {
 
+// `grid` workgroups of `wpb` wavefronts; a.lds_per_wave bytes of LDS per wavefront.
 template <typename TS, int NMAX, int MIC>
-hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int lds, hipStream_t st, int relax) {
-  hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC>), dim3(grid), dim3(64), lds, st, a, relax);
+hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st, int relax) {
+  hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a, relax);
   return hipGetLastError();
 }
 
 template <typename TS, int NMAX, int MIC>
-hipError_t launch_wave_bwd(const Args<TS>& a, unsigned grid, int lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_wave_bwd<TS, NMAX, MIC>), dim3(grid), dim3(64), lds, st, a);
+hipError_t launch_wave_bwd(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st) {
+  hipLaunchKernelGGL((k_wave_bwd<TS, NMAX, MIC>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a);
   return hipGetLastError();
+}
+
+// Resident workgroups of `wpb` wavefronts per CU (forward kernel, the longer-running of the two).
+template <typename TS, int NMAX, int MIC>
+int wave_occupancy(int wpb, int lds_bytes) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_wave_solve<TS, NMAX, MIC>),
+                                                   64 * wpb, (size_t)lds_bytes * wpb) != hipSuccess)
+    nb = 0;
+  return nb;
 }
 
 template hipError_t launch_wave_solve<QPX_TS, QPX_NMAX, QPX_MIC>(const Args<QPX_TS>&, unsigned, int, hipStream_t, int);
+template int wave_occupancy<QPX_TS, QPX_NMAX, QPX_MIC>(int, int);
 template hipError_t launch_wave_bwd<QPX_TS, QPX_NMAX, QPX_MIC>(const Args<QPX_TS>&, unsigned, int, hipStream_t);
 
 }  // namespace wave

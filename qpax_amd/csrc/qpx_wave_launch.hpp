@@ -30,9 +30,9 @@ __host__ __device__ inline Lds lds_layout(int n, int me, int mi, int ts) {
 }
 
 template <typename TS, int NMAX, int MIC>
-hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int lds, hipStream_t st, int relax);
+hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st, int relax);
 template <typename TS, int NMAX, int MIC>
-hipError_t launch_wave_bwd(const Args<TS>& a, unsigned grid, int lds, hipStream_t st);
+hipError_t launch_wave_bwd(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st);
 
 }  // namespace wave
 }  // namespace qpx

@@ -1,0 +1,178 @@
+"""Ragged batches (mixed n, m_eq, m_ineq in one call): parity with the uniform entry points, with
+the float64 oracle per problem, across autotuning, and under hipGraph capture and replay."""
+
+import numpy as np
+import pytest
+
+from oracle import qp_oracle as O
+from qpax_amd import problems as P
+from tests._util import rel_err, to_np
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+NAMES = ("Q", "q", "A", "b", "G", "h")
+
+
+def _plan(probs, dt):
+    from qpax_amd.ragged import RaggedPlan
+
+    n = [p[1].shape[0] for p in probs]
+    me = [p[3].shape[0] for p in probs]
+    mi = [p[5].shape[0] for p in probs]
+    plan = RaggedPlan(n, me, mi, dtype=dt)
+    return plan, plan.pack(probs)
+
+
+def _uniform(prob, dt, tol, kappa):
+    """One problem through the uniform fused entry points (forward + relax, then backward)."""
+    from qpax_amd.diff_qp import diff_qp_batch
+    from qpax_amd.pdip import solve_qp_relaxed_batch
+    from qpax_amd._tensors import normalize
+
+    bt = normalize(*[torch.tensor(np.asarray(v)[None], dtype=dt, device="cuda") for v in prob])
+    out = solve_qp_relaxed_batch(bt, tol, kappa, mode="fused")
+    xr, sr, zr, yr = out[6], out[7], out[8], out[9]
+    g = diff_qp_batch(bt, (xr, sr, zr, yr), 2.0 * (out[0] - 1.0), mode="fused")
+    return out, g
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_ragged_matches_uniform_bitwise(dt):
+    """Each problem of a ragged batch gets exactly the arithmetic of a uniform call of its size."""
+    probs = P.uneven(40, seed=41, n_lo=8, n_hi=128)
+    print(f"[case] ragged vs uniform dt={dt} B=40", flush=True)
+    plan, ops = _plan(probs, dt)
+    out = plan.solve_relaxed(ops, 1e-5, 1e-3)
+    grads = plan.diff(ops, out[6:10], 2.0 * (out[0] - 1.0))
+    torch.cuda.synchronize()
+    kinds = {b["kind"] for b in plan.buckets()}
+    assert kinds == {"wave", "block"}
+    for i in range(0, 40, 3):
+        uo, ug = _uniform(probs[i], dt, 1e-5, 1e-3)
+        for k, which in enumerate(("q", "h", "h", "b")):
+            a = to_np(plan.unpack(out[k], which, i))
+            b = to_np(uo[k])[0]
+            assert np.array_equal(a, b), (i, k)
+            a = to_np(plan.unpack(out[6 + k], which, i))
+            b = to_np(uo[6 + k])[0]
+            assert np.array_equal(a, b), (i, 6 + k)
+        for k in (4, 5, 10, 11):
+            assert int(to_np(out[k])[i]) == int(to_np(uo[k])[0]), (i, k)
+        for k, which in enumerate(NAMES):
+            a = to_np(plan.unpack(grads[k], which, i))
+            b = to_np(ug[k])[0]
+            assert np.array_equal(a, b), (i, which)
+
+
+def test_ragged_matches_oracle_f64():
+    probs = P.uneven(24, seed=42, n_lo=8, n_hi=96)
+    print("[case] ragged vs oracle f64 B=24", flush=True)
+    plan, ops = _plan(probs, torch.float64)
+    out = plan.solve_relaxed(ops, 1e-5, 1e-3)
+    x = out[0]
+    grads = plan.diff(ops, out[6:10], 2.0 * (x - 1.0))
+    torch.cuda.synchronize()
+    for i, pr in enumerate(probs):
+        d = [np.asarray(v, np.float64)[None] for v in pr]
+        xo, rel, info = O.solve_qp_primal_fwd(*d, solver_tol=1e-5, target_kappa=1e-3)
+        go = O.solve_qp_primal_bwd(*d, rel, 2 * (xo - 1))
+        assert int(to_np(out[4])[i]) == int(info[0][0])
+        assert abs(int(to_np(out[5])[i]) - int(info[1][0])) <= 1
+        if info[0][0] != 1 or info[2][0] != 1:
+            continue
+        assert rel_err(to_np(plan.unpack(x, "q", i))[None], xo).max() < 1e-7
+        assert rel_err(to_np(plan.unpack(out[6], "q", i))[None], rel[0]).max() < 1e-7
+        for k, which in enumerate(NAMES):
+            if go[k].size == 0:
+                continue
+            assert rel_err(to_np(plan.unpack(grads[k], which, i))[None], go[k]).max() < 1e-6, (i, which)
+
+
+def test_tune_keeps_results():
+    probs = P.uneven(64, seed=43, n_lo=8, n_hi=128)
+    print("[case] ragged tune f64 B=64", flush=True)
+    plan, ops = _plan(probs, torch.float64)
+    ref = [t.clone() for t in plan.solve_relaxed(ops, 1e-5, 1e-3)]
+    tuned = plan.tune(ops, 1e-5, 1e-3)
+    again = plan.solve_relaxed(ops, 1e-5, 1e-3)
+    torch.cuda.synchronize()
+    for rows in plan.buckets():
+        print(rows)
+    for a, b in zip(tuned, again):
+        assert torch.equal(a, b)
+    # a different launch shape may reorder reductions: rounding-level differences, which can move
+    # the convergence test by one iteration
+    it_a, it_b = to_np(ref[5]), to_np(tuned[5])
+    assert np.array_equal(to_np(ref[4]), to_np(tuned[4]))
+    assert np.abs(it_a.astype(int) - it_b).max() <= 1
+    same = it_a == it_b
+    for i in range(len(probs)):
+        e = rel_err(to_np(plan.unpack(ref[0], "q", i))[None], to_np(plan.unpack(tuned[0], "q", i))[None])[0]
+        assert e < (1e-9 if same[i] else 1e-4), (i, e)
+
+
+def test_graph_capture_replays_match_eager():
+    probs = P.uneven(48, seed=44, n_lo=8, n_hi=128)
+    print("[case] ragged hipGraph f32 B=48", flush=True)
+    plan, ops = _plan(probs, torch.float32)
+    eager = plan.solve_relaxed(ops, 1e-5, 1e-3)
+    dl = (2.0 * (eager[0] - 1.0)).contiguous()
+    eg = plan.diff(ops, eager[6:10], dl)
+    eager = [t.clone() for t in eager]
+    eg = [t.clone() for t in eg]
+    g, fwd, grads = plan.capture(ops, dl, 1e-5, 1e-3)
+    for rep in range(3):
+        for t in list(fwd) + list(grads):
+            t.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(fwd, eager):
+            assert torch.equal(a, b), rep
+        for a, b in zip(grads, eg):
+            assert torch.equal(a, b), rep
+        # unrelated eager work between replays
+        (torch.randn(1024, device="cuda") * 2).sum().item()
+
+
+def test_uneven_full_size_f32():
+    """16384 mixed problems (n ~ U{8..128}, m_ineq ~ U{n..3n}, m_eq ~ U{0..n/4}, seed 4), float32
+    forward + backward in one tuned plan; a sample checked per problem against the float64 oracle."""
+    probs = P.uneven(16384, seed=4)
+    print("[case] uneven full size f32 B=16384", flush=True)
+    plan, ops = _plan(probs, torch.float32)
+    plan.tune(ops, 1e-5, 1e-3)
+    out = plan.solve_relaxed(ops, 1e-5, 1e-3)
+    x = out[0]
+    grads = plan.diff(ops, out[6:10], 2.0 * (x - 1.0))
+    torch.cuda.synchronize()
+    conv, rconv = to_np(out[4]), to_np(out[10])
+    print("converged", conv.mean(), "relax converged", rconv.mean())
+    for g in grads:
+        assert torch.isfinite(g).all()
+    rng = np.random.default_rng(0)
+    errs, gerrs = [], []
+    for i in rng.choice(16384, 12, replace=False):
+        pr = probs[i]
+        d = [np.asarray(np.asarray(v, np.float32), np.float64)[None] for v in pr]
+        xo, rel, info = O.solve_qp_primal_fwd(*d, solver_tol=1e-5, target_kappa=1e-3)
+        if info[0][0] != 1 or info[2][0] != 1:
+            continue
+        go = O.solve_qp_primal_bwd(*d, rel, 2 * (xo - 1))
+        errs.append(rel_err(to_np(plan.unpack(x, "q", i)).astype(np.float64)[None], xo).max())
+        for k, which in enumerate(NAMES):
+            if go[k].size:
+                gerrs.append(rel_err(to_np(plan.unpack(grads[k], which, i)).astype(np.float64)[None], go[k]).max())
+    print(f"sample x err max {max(errs):.2e}, grad err max {max(gerrs):.2e}")
+    assert len(errs) >= 8
+    assert max(errs) < 1e-4
+    assert max(gerrs) < 1e-3
+
+
+def test_plan_rejects_bad_sizes():
+    from qpax_amd.ragged import RaggedPlan
+
+    with pytest.raises(RuntimeError, match="invalid sizes"):
+        RaggedPlan([4, 0], [0, 0], [3, 3])
+    with pytest.raises(RuntimeError, match="invalid sizes"):
+        RaggedPlan([4, 5], [0, 0], [3, 0])
