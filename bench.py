@@ -133,6 +133,86 @@ def cpu_baseline(data, sample, xg_gpu):
     return sample / dt, xo, go, info
 
 
+def uneven_bench(torch, args):
+    """Uneven batching: 16384 QPs with n ~ U{8..128}, m_ineq ~ U{n..3n}, m_eq ~ U{0..n/4} (seed 4),
+    float32 fwd (solve + relax) + bwd through one autotuned ragged plan, captured in a CUDA graph
+    and replayed; one step = one replay."""
+    from qpax_amd import problems as P
+    from qpax_amd.ragged import RaggedPlan
+
+    B = args.uneven_batch
+    probs = P.uneven(B, seed=4)
+    n = np.array([p[1].shape[0] for p in probs])
+    me = np.array([p[3].shape[0] for p in probs])
+    mi = np.array([p[5].shape[0] for p in probs])
+    plan = RaggedPlan(n, me, mi, dtype=torch.float32)
+    ops = plan.pack(probs)
+    t0 = time.perf_counter()
+    plan.tune(ops, TOL, KAPPA)
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t0
+    out = plan.solve_relaxed(ops, TOL, KAPPA)
+    dl = (2.0 * (out[0] - 1.0)).contiguous()
+    g, fwd, grads = plan.capture(ops, dl, TOL, KAPPA)
+    ms = 1e3 * time_steps(torch, g.replay, args.steps, args.warmup)
+    # forward alone (eager) for the roofline of the solve + relax launches
+    st = torch.cuda.current_stream()
+    f_ms = []
+    for r in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        plan.solve_relaxed(ops, TOL, KAPPA, out=fwd)
+        e1.record(st)
+        torch.cuda.synchronize()
+        if r:
+            f_ms.append(e0.elapsed_time(e1))
+    f_ms = float(np.mean(f_ms))
+    it_f = fwd[5].cpu().numpy().astype(np.float64)
+    it_r = fwd[11].cpu().numpy().astype(np.float64)
+    fl = sum(flops_per_qp(a, b, c, d, e, bwd=False)[0] for a, b, c, d, e in zip(n, me, mi, it_f, it_r))
+    achieved = fl / (f_ms * 1e-3) / 1e12
+    res = {
+        "metric": "fwd+bwd QPs/s, uneven batch (hipGraph-captured ragged plan)",
+        "value": round(B / (ms * 1e-3), 1), "unit": "QPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{B} QPs, n~U{{8..128}}, m_ineq~U{{n..3n}}, m_eq~U{{0..n/4}}, seed 4, float32 "
+                               "inputs/outputs, float64 arithmetic, solve_qp_primal fwd+bwd, tol 1e-5, kappa 1e-3",
+                   "batch": B, "io_dtype": "f32"},
+        "tune_s": round(tune_s, 2),
+        "fwd_ms": round(f_ms, 3),
+        "converged_frac": float((fwd[4] == 1).float().mean()),
+        "relax_converged_frac": float((fwd[10] == 1).float().mean()),
+        "buckets": plan.buckets(),
+        "roofline": {"kernel": "all buckets of the forward (solve + relax)", "bound": "mfma",
+                     "achieved": round(achieved, 3), "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": None,
+                     "algorithmic_flops_per_launch": fl},
+    }
+    if not args.no_cpu:
+        from oracle import qp_oracle as O
+
+        rng = np.random.default_rng(0)
+        idx = rng.choice(B, args.uneven_check, replace=False)
+        ex, eg = [], []
+        x = fwd[0]
+        for i in idx:
+            d = [np.asarray(np.asarray(v, np.float32), np.float64)[None] for v in probs[i]]
+            xo, rel, info = O.solve_qp_primal_fwd(*d, solver_tol=TOL, target_kappa=KAPPA)
+            if info[0][0] != 1 or info[2][0] != 1:
+                continue
+            go = O.solve_qp_primal_bwd(*d, rel, 2 * (xo - 1))
+            xg = plan.unpack(x, "q", int(i)).cpu().numpy().astype(np.float64)
+            ex.append(np.linalg.norm(xg - xo[0]) / max(np.linalg.norm(xo[0]), 1e-12))
+            for k, w in enumerate("Q q A b G h".split()):
+                if go[k].size:
+                    gg = plan.unpack(grads[k], w, int(i)).cpu().numpy().astype(np.float64)
+                    eg.append(np.linalg.norm(gg - go[k][0]) / max(np.linalg.norm(go[k][0]), 1e-12))
+        res["x_rel_err_vs_f64_oracle"] = {"max": float(max(ex)), "median": float(np.median(ex)), "checked": len(ex)}
+        res["grad_rel_err_vs_f64_oracle"] = {"max": float(max(eg)), "median": float(np.median(eg))}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +223,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only run a few fused steps (for rocprofv3)")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a plain device copy (counter calibration)")
+    ap.add_argument("--workload", choices=("c3", "uneven"), default="c3",
+                    help="c3: the headline n=50 B=4096 workload; uneven: 16384 mixed-size QPs (ragged plan)")
+    ap.add_argument("--uneven-batch", type=int, default=16384)
+    ap.add_argument("--uneven-check", type=int, default=32, help="problems of the uneven batch checked vs the oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
@@ -161,6 +245,10 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes": nb}))
+        return
+
+    if args.workload == "uneven":
+        uneven_bench(torch, args)
         return
 
     data, ts = make_inputs(torch, dev)

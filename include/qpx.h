@@ -170,9 +170,10 @@ QPX_API qpx_status qpx_plan_info(const qpx_plan* plan, int64_t* info);
 QPX_API size_t qpx_plan_workspace_size(const qpx_plan* plan);
 /* offsets (batch x 6, host): element offsets of problem i in packed Q, q, A, b, G, h. */
 QPX_API qpx_status qpx_plan_offsets(const qpx_plan* plan, int64_t* offsets);
-/* out[0..9] of bucket i: kind (0 wavefront kernel, 1 workgroup kernel), NMAX, MIC, threads,
- * wavefronts per workgroup, persistent, problems, LDS bytes per wavefront, workgroups, flops per
- * iteration. */
+/* out[0..9] of bucket i: kind (0 wavefront kernel, 1 generic workgroup kernel, 2 matrix-core
+ * workgroup kernel), NMAX (kind 0) or tiles per side (kind 2), MIC, threads, wavefronts per
+ * workgroup, persistent, problems, LDS bytes per wavefront (kind 0) or workgroup (kind 2),
+ * workgroups, flops per iteration. */
 QPX_API qpx_status qpx_plan_bucket(const qpx_plan* plan, int32_t i, int64_t* out);
 /* qpx_solve_qp over a ragged batch (packed operands and outputs). */
 QPX_API qpx_status qpx_plan_solve(const qpx_plan* plan, const void* Q, const void* q, const void* A,

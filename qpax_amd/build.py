@@ -38,6 +38,10 @@ WAVE_INSTANCES = [(ts, nmax, mic) for ts in ("float", "double")
                   for (nmax, mic) in ((32, 1), (32, 2), (64, 1), (64, 2), (64, 4))]
 
 
+# (storage type, NTI, W) instantiations of the workgroup-per-QP matrix-core kernels
+GROUP_INSTANCES = [(ts, nti, w) for ts in ("float", "double") for nti in range(4, 11) for w in (4, 8)]
+
+
 def _units():
     inc = ["-I", os.path.join(HERE, "..", "include")]
     units = [(os.path.join(CSRC, "qpx_solver.hip"), os.path.join(OBJ, "qpx_solver.o"), inc),
@@ -46,6 +50,10 @@ def _units():
         defs = [f"-DQPX_TS={ts}", f"-DQPX_NMAX={nmax}", f"-DQPX_MIC={mic}"]
         units.append((os.path.join(CSRC, "qpx_wave_inst.hip"),
                       os.path.join(OBJ, f"qpx_wave_{ts[0]}_{nmax}_{mic}.o"), inc + defs))
+    for ts, nti, w in GROUP_INSTANCES:
+        defs = [f"-DQPX_TS={ts}", f"-DQPX_NTI={nti}", f"-DQPX_W={w}"]
+        units.append((os.path.join(CSRC, "qpx_group_inst.hip"),
+                      os.path.join(OBJ, f"qpx_group_{ts[0]}_{nti}_{w}.o"), inc + defs))
     return units
 
 

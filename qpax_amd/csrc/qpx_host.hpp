@@ -38,6 +38,14 @@ struct WaveVariant {
 };
 bool wave_variant(int n, int me, int mi, int ts, WaveVariant* v);
 
+// Workgroup-per-QP matrix-core kernel instance (NTI 16-row tiles per side of the augmented KKT
+// matrix, W wavefronts), or false if the problem does not fit one.  `w` = 0 picks the default.
+struct GroupVariant {
+  int nti, w;
+};
+bool group_variant(int n, int me, int mi, int ts, int w, GroupVariant* v);
+int group_lds_bytes(int n, int me, int mi, int ts, GroupVariant v);
+
 }  // namespace host
 
 // Launchers of the workgroup-per-QP kernels (any size), NT in {64, 128, 256}.
@@ -47,6 +55,46 @@ template <int NT, typename TS>
 hipError_t launch_generic_bwd(const Args<TS>& a, unsigned grid, hipStream_t st);
 template <int NT, typename TS>
 int generic_occupancy();  // resident workgroups per CU
+
+namespace group {
+template <typename TS, int NTI, int W>
+hipError_t launch_group_solve(const Args<TS>& a, unsigned grid, int lds, hipStream_t st, int relax);
+template <typename TS, int NTI, int W>
+hipError_t launch_group_bwd(const Args<TS>& a, unsigned grid, int lds, hipStream_t st);
+template <typename TS, int NTI, int W>
+int group_occupancy(int lds);
+
+// Runtime selection among the compiled (NTI, W) instances: NTI in 4..10, W in {4, 8}.
+#define QPX_GROUP_SWITCH(V, CALL)                                            \
+  switch (V.nti * 16 + V.w) {                                                \
+    case 4 * 16 + 4: return CALL(4, 4);   case 4 * 16 + 8: return CALL(4, 8);   \
+    case 5 * 16 + 4: return CALL(5, 4);   case 5 * 16 + 8: return CALL(5, 8);   \
+    case 6 * 16 + 4: return CALL(6, 4);   case 6 * 16 + 8: return CALL(6, 8);   \
+    case 7 * 16 + 4: return CALL(7, 4);   case 7 * 16 + 8: return CALL(7, 8);   \
+    case 8 * 16 + 4: return CALL(8, 4);   case 8 * 16 + 8: return CALL(8, 8);   \
+    case 9 * 16 + 4: return CALL(9, 4);   case 9 * 16 + 8: return CALL(9, 8);   \
+    case 10 * 16 + 4: return CALL(10, 4); default: return CALL(10, 8);          \
+  }
+
+template <typename TS>
+inline hipError_t launch_solve(host::GroupVariant v, const Args<TS>& a, unsigned grid, int lds, hipStream_t st, int relax) {
+#define QPX_C(N, W) launch_group_solve<TS, N, W>(a, grid, lds, st, relax)
+  QPX_GROUP_SWITCH(v, QPX_C)
+#undef QPX_C
+}
+template <typename TS>
+inline hipError_t launch_bwd(host::GroupVariant v, const Args<TS>& a, unsigned grid, int lds, hipStream_t st) {
+#define QPX_C(N, W) launch_group_bwd<TS, N, W>(a, grid, lds, st)
+  QPX_GROUP_SWITCH(v, QPX_C)
+#undef QPX_C
+}
+template <typename TS>
+inline int occupancy(host::GroupVariant v, int lds) {
+#define QPX_C(N, W) group_occupancy<TS, N, W>(lds)
+  QPX_GROUP_SWITCH(v, QPX_C)
+#undef QPX_C
+}
+}  // namespace group
 
 namespace wave {
 template <typename TS, int NMAX, int MIC>

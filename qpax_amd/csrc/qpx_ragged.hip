@@ -31,14 +31,15 @@ __global__ void k_fill_u32(unsigned* p, int count, unsigned v) {
   for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = v;
 }
 
-enum Kind { kWave = 0, kBlock = 1 };
+enum Kind { kWave = 0, kBlock = 1, kGroup = 2 };
 
 struct Bucket {
   int kind;
   qpx::host::WaveVariant wv;  // kWave: kernel instance
+  qpx::host::GroupVariant gv; // kGroup: kernel instance
   int nt;                     // kBlock: threads per workgroup
   int wpb;                    // kWave: wavefronts per workgroup
-  int lds_per_wave;           // kWave: largest LDS image among the bucket's problems
+  int lds_per_wave;           // kWave / kGroup: largest LDS image among the bucket's problems
   int persistent;             // 1: grid of resident slots pulling from the queue; 0: one slot per problem
   int64_t first, count;       // range of the device order array
   double flops;               // per-iteration flop estimate of the whole bucket
@@ -69,6 +70,7 @@ struct qpx_plan {
   int64_t totals[6] = {0, 0, 0, 0, 0, 0};  // elements of Q, q, A, b, G, h
   int64_t ws_words = 0;
   std::vector<Bucket> buckets;
+  std::vector<int> order;  // host copy of the device order
   RaggedEntry* d_table = nullptr;
   int* d_order = nullptr;
 };
@@ -99,6 +101,8 @@ unsigned grid_of(const Bucket& b) {
   int per_cu = 0;
   if (b.kind == kWave) {
     per_cu = qpx::wave::occupancy<TS>(b.wv, b.wpb, b.lds_per_wave);
+  } else if (b.kind == kGroup) {
+    per_cu = qpx::group::occupancy<TS>(b.gv, b.lds_per_wave);
   } else {
     per_cu = b.nt == 64 ? qpx::generic_occupancy<64, TS>() : b.nt == 128 ? qpx::generic_occupancy<128, TS>()
                                                                           : qpx::generic_occupancy<256, TS>();
@@ -114,6 +118,7 @@ template <typename TS>
 hipError_t launch_bucket_fwd(const qpx::Args<TS>& a, const Bucket& b, hipStream_t st, int relax) {
   const unsigned grid = grid_of<TS>(b);
   if (b.kind == kWave) return qpx::wave::launch_solve(b.wv, a, grid, b.wpb, st, relax);
+  if (b.kind == kGroup) return qpx::group::launch_solve(b.gv, a, grid, b.lds_per_wave, st, relax);
   if (b.nt == 64) return qpx::launch_generic_solve<64, TS>(a, grid, st, relax);
   if (b.nt == 128) return qpx::launch_generic_solve<128, TS>(a, grid, st, relax);
   return qpx::launch_generic_solve<256, TS>(a, grid, st, relax);
@@ -123,6 +128,7 @@ template <typename TS>
 hipError_t launch_bucket_bwd(const qpx::Args<TS>& a, const Bucket& b, hipStream_t st) {
   const unsigned grid = grid_of<TS>(b);
   if (b.kind == kWave) return qpx::wave::launch_bwd(b.wv, a, grid, b.wpb, st);
+  if (b.kind == kGroup) return qpx::group::launch_bwd(b.gv, a, grid, b.lds_per_wave, st);
   if (b.nt == 64) return qpx::launch_generic_bwd<64, TS>(a, grid, st);
   if (b.nt == 128) return qpx::launch_generic_bwd<128, TS>(a, grid, st);
   return qpx::launch_generic_bwd<256, TS>(a, grid, st);
@@ -272,8 +278,11 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
   auto key_of = [&](const RaggedEntry& e, Bucket* b) {
     std::memset(b, 0, sizeof *b);
     qpx::host::WaveVariant wv;
+    qpx::host::GroupVariant gv;
     if (qpx::host::wave_variant(e.n, e.me, e.mi, ts, &wv)) {
       b->kind = kWave; b->wv = wv; b->wpb = 1;
+    } else if (qpx::host::group_variant(e.n, e.me, e.mi, ts, 0, &gv)) {
+      b->kind = kGroup; b->gv = gv;
     } else {
       b->kind = kBlock; b->nt = block_threads(e.n, e.mi);
     }
@@ -283,7 +292,9 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
     key_of(p->table[i], &k);
     size_t j = 0;
     for (; j < bks.size(); ++j)
-      if (bks[j].kind == k.kind && bks[j].nt == k.nt && bks[j].wv.nmax == k.wv.nmax && bks[j].wv.mic == k.wv.mic) break;
+      if (bks[j].kind == k.kind && bks[j].nt == k.nt && bks[j].wv.nmax == k.wv.nmax && bks[j].wv.mic == k.wv.mic &&
+          bks[j].gv.nti == k.gv.nti && bks[j].gv.w == k.gv.w)
+        break;
     if (j == bks.size()) { bks.push_back(k); members.emplace_back(); }
     members[j].push_back((int)i);
   }
@@ -303,6 +314,7 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
       cost[k] = iter_flops(e.n, e.me, e.mi);
       total += cost[k];
       if (bks[j].kind == kWave) lds = std::max(lds, qpx::wave::lds_layout(e.n, e.me, e.mi, ts).bytes);
+      if (bks[j].kind == kGroup) lds = std::max(lds, qpx::host::group_lds_bytes(e.n, e.me, e.mi, ts, bks[j].gv));
     }
     std::vector<size_t> idx(mem.size());
     std::iota(idx.begin(), idx.end(), 0);
@@ -319,6 +331,7 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
   std::iota(bo.begin(), bo.end(), 0);
   std::stable_sort(bo.begin(), bo.end(), [&](size_t a, size_t b) { return bks[a].flops > bks[b].flops; });
   for (size_t k : bo) p->buckets.push_back(bks[k]);
+  p->order = order;
 
   if (batch > 0) {
     if (hipMalloc(reinterpret_cast<void**>(&p->d_table), sizeof(RaggedEntry) * batch) != hipSuccess ||
@@ -371,7 +384,8 @@ QPX_API qpx_status qpx_plan_bucket(const qpx_plan* p, int32_t i, int64_t* out) {
   if (!p || !out) return fail(QPX_ERR_ARG, "null argument");
   if (i < 0 || i >= (int)p->buckets.size()) return fail(QPX_ERR_ARG, "bucket %d out of range", i);
   const Bucket& b = p->buckets[i];
-  out[0] = b.kind; out[1] = b.wv.nmax; out[2] = b.wv.mic; out[3] = b.nt; out[4] = b.wpb; out[5] = b.persistent;
+  out[0] = b.kind; out[1] = b.kind == kGroup ? b.gv.nti : b.wv.nmax; out[2] = b.wv.mic;
+  out[3] = b.kind == kGroup ? 64 * b.gv.w : b.nt; out[4] = b.kind == kGroup ? b.gv.w : b.wpb; out[5] = b.persistent;
   out[6] = b.count; out[7] = b.lds_per_wave;
   out[8] = p->dtype == QPX_F32 ? grid_of<float>(b) : grid_of<double>(b);
   out[9] = (int64_t)b.flops;
@@ -436,6 +450,7 @@ QPX_API qpx_status qpx_plan_tune(qpx_plan* p, const void* Q, const void* q, cons
   if ((st = check_fwd(p, io, true)) != QPX_OK || p->B == 0) return st;
   hipStream_t hs = (hipStream_t)stream;
   const int ts = p->dtype == QPX_F32 ? 4 : 8;
+  const std::vector<int>& host_order = p->order;
   hipEvent_t e0, e1;
   QPX_HIP(hipEventCreate(&e0));
   QPX_HIP(hipEventCreate(&e1));
@@ -448,6 +463,21 @@ QPX_API qpx_status qpx_plan_tune(qpx_plan* p, const void* Q, const void* q, cons
           if ((int64_t)wpb * bk.lds_per_wave > qpx::host::max_lds_per_block()) continue;
           Bucket c = bk;
           c.wpb = wpb;
+          c.persistent = pers;
+          cands.push_back(c);
+        }
+    } else if (bk.kind == kGroup) {
+      for (int w : {4, 8})
+        for (int pers : {1, 0}) {
+          Bucket c = bk;
+          c.gv.w = w;
+          int lds = 0;
+          for (int64_t k = 0; k < bk.count; ++k) {
+            const RaggedEntry& e = p->table[host_order[bk.first + k]];
+            lds = std::max(lds, qpx::host::group_lds_bytes(e.n, e.me, e.mi, ts, c.gv));
+          }
+          if (lds + 256 > qpx::host::max_lds_per_block()) continue;
+          c.lds_per_wave = lds;
           c.persistent = pers;
           cands.push_back(c);
         }

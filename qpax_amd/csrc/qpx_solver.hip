@@ -20,12 +20,8 @@
 #include "qpx_core.hpp"
 #include "qpx_elastic.hpp"
 
-namespace qpx {
-
-
-
-}  // namespace qpx
 #include "qpx_wave_launch.hpp"
+#include "qpx_group.hpp"
 namespace qpx {
 
 template <typename TS>
@@ -469,8 +465,10 @@ const char* last_error() { return g_err.c_str(); }
 // KKT matrix (m = n padded to a multiple of 4, plus m_eq), whichever is larger.
 int64_t stride_words(int n, int me, int mi) {
   const int64_t m = qpx::wave::padded_n(n) + me;
-  const int64_t w = qpx::elastic_words(n, me, mi);
-  return w > qpx::round_up(m * m, 64) ? w : qpx::round_up(m * m, 64);
+  int64_t w = qpx::elastic_words(n, me, mi);
+  if (w < qpx::round_up(m * m, 64)) w = qpx::round_up(m * m, 64);
+  const int64_t g = qpx::group::group_words(n, me, mi);
+  return w > g ? w : g;
 }
 
 int max_lds_per_block() {
@@ -503,6 +501,22 @@ bool wave_variant(int n, int me, int mi, int ts, WaveVariant* v) {
   else if (mi <= 128) *v = {64, 2};
   else *v = {64, 4};
   return true;
+}
+
+bool group_variant(int n, int me, int mi, int ts, int w, GroupVariant* v) {
+  const int m = qpx::wave::padded_n(n) + me;
+  int nti = qpx::group::ntiles(m);
+  if (nti > 10 || mi > 512) return false;
+  if (nti < 4) nti = 4;
+  if (w != 4 && w != 8) w = 4;
+  *v = {nti, w};
+  return group_lds_bytes(n, me, mi, ts, *v) + 256 <= max_lds_per_block();  // + the kernel's static LDS
+}
+
+int group_lds_bytes(int n, int me, int mi, int ts, GroupVariant v) {
+  const int m = qpx::wave::padded_n(n) + me;
+  (void)v.nti;
+  return qpx::group::group_lds(m, mi, 64 * v.w, ts).bytes;
 }
 
 }  // namespace host
@@ -566,6 +580,12 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
       QPX_HIP(qpx::wave::launch_solve(wv, a, (unsigned)a.B, 1, st, relax ? 1 : 0));
       return QPX_OK;
     }
+    qpx::host::GroupVariant gv;
+    if (qpx::host::group_variant(a.n, a.me, a.mi, (int)sizeof(TS), 0, &gv)) {
+      const int lds = qpx::host::group_lds_bytes(a.n, a.me, a.mi, (int)sizeof(TS), gv);
+      QPX_HIP(qpx::group::launch_solve(gv, a, (unsigned)a.B, lds, st, relax ? 1 : 0));
+      return QPX_OK;
+    }
     hipLaunchKernelGGL((qpx::k_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
     QPX_HIP(hipGetLastError());
     return QPX_OK;
@@ -605,6 +625,12 @@ qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
     qpx::host::WaveVariant wv;
     if (wave_plan(a, &wv)) {
       QPX_HIP(qpx::wave::launch_bwd(wv, a, (unsigned)a.B, 1, st));
+      return QPX_OK;
+    }
+    qpx::host::GroupVariant gv;
+    if (qpx::host::group_variant(a.n, a.me, a.mi, (int)sizeof(TS), 0, &gv)) {
+      const int lds = qpx::host::group_lds_bytes(a.n, a.me, a.mi, (int)sizeof(TS), gv);
+      QPX_HIP(qpx::group::launch_bwd(gv, a, (unsigned)a.B, lds, st));
       return QPX_OK;
     }
     hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, 0, st, a);

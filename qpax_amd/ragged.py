@@ -158,7 +158,8 @@ class RaggedPlan:
         for i in range(self.n_buckets):
             o = np.zeros(10, dtype=np.int64)
             _lib.check(self._lib.qpx_plan_bucket(self._h, i, o.ctypes.data), "qpx_plan_bucket")
-            rows.append(dict(kind="wave" if o[0] == 0 else "block", nmax=int(o[1]), mic=int(o[2]), threads=int(o[3]),
+            kind = {0: "wave", 1: "block", 2: "group"}[int(o[0])]
+            rows.append(dict(kind=kind, nmax=int(o[1]), mic=int(o[2]), threads=int(o[3]),
                              waves_per_group=int(o[4]), persistent=bool(o[5]), problems=int(o[6]),
                              lds_per_wave=int(o[7]), groups=int(o[8]), flops_per_iter=int(o[9])))
         return rows

@@ -41,13 +41,17 @@ def _uniform(prob, dt, tol, kappa):
 def test_ragged_matches_uniform_bitwise(dt):
     """Each problem of a ragged batch gets exactly the arithmetic of a uniform call of its size."""
     probs = P.uneven(40, seed=41, n_lo=8, n_hi=128)
+    # two problems beyond the matrix-core kernels (m > 160): the generic workgroup kernel
+    for sd in (1, 2):
+        Q, q, A, b, G, h = P.random_feasible(1, 150, 20, 200, seed=sd)
+        probs[3 * sd] = (Q[0], q[0], A[0], b[0], G[0], h[0])
     print(f"[case] ragged vs uniform dt={dt} B=40", flush=True)
     plan, ops = _plan(probs, dt)
     out = plan.solve_relaxed(ops, 1e-5, 1e-3)
     grads = plan.diff(ops, out[6:10], 2.0 * (out[0] - 1.0))
     torch.cuda.synchronize()
     kinds = {b["kind"] for b in plan.buckets()}
-    assert kinds == {"wave", "block"}
+    assert kinds == {"wave", "group", "block"}
     for i in range(0, 40, 3):
         uo, ug = _uniform(probs[i], dt, 1e-5, 1e-3)
         for k, which in enumerate(("q", "h", "h", "b")):
@@ -101,7 +105,8 @@ def test_tune_keeps_results():
         print(rows)
     for a, b in zip(tuned, again):
         assert torch.equal(a, b)
-    # a different launch shape may reorder reductions: rounding-level differences, which can move
+    # a different launch shape may reorder reductions: rounding-level differences (amplified to
+    # ~1e-9 by the conditioning of n >= 70 problems at the stopping point), which can also move
     # the convergence test by one iteration
     it_a, it_b = to_np(ref[5]), to_np(tuned[5])
     assert np.array_equal(to_np(ref[4]), to_np(tuned[4]))
@@ -109,7 +114,7 @@ def test_tune_keeps_results():
     same = it_a == it_b
     for i in range(len(probs)):
         e = rel_err(to_np(plan.unpack(ref[0], "q", i))[None], to_np(plan.unpack(tuned[0], "q", i))[None])[0]
-        assert e < (1e-9 if same[i] else 1e-4), (i, e)
+        assert e < (3e-9 if same[i] else 1e-4), (i, e)
 
 
 def test_graph_capture_replays_match_eager():

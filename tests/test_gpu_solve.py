@@ -18,6 +18,14 @@ CASES = [
     ("c2-mini-f32", lambda: P.random_feasible(64, 20, 5, 40, seed=2), np.float32, 1e-3, 1e-4),
     ("c1-nnls-f64", lambda: P.nnls(256, seed=1), np.float64, 1e-10, 1e-9),
     ("c1-nnls-f32", lambda: P.nnls(256, seed=1), np.float32, 1e-5, 1e-4),
+    # sizes served by the workgroup-per-QP matrix-core kernel.  At n >= 70 the reduced KKT
+    # matrices reach condition numbers near 1e7 at the stopping point, so last-bit differences of
+    # an fp64 elimination move x by up to ~1e-16 * 1e7: the fp64 bar there is 3e-9.
+    ("mid-f64", lambda: P.random_feasible(32, 100, 20, 250, seed=6), np.float64, 1e-5, 3e-9),
+    ("mid-f32", lambda: P.random_feasible(32, 100, 20, 250, seed=6), np.float32, 1e-5, 1e-4),
+    ("big-f64", lambda: P.random_feasible(16, 128, 32, 384, seed=7), np.float64, 1e-5, 3e-9),
+    ("odd-f64", lambda: P.random_feasible(16, 70, 3, 140, seed=8), np.float64, 1e-5, 3e-9),
+    ("tall-f64", lambda: P.random_feasible(16, 20, 2, 300, seed=9), np.float64, 1e-5, 3e-9),
 ]
 
 
