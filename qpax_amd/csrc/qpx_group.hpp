@@ -39,20 +39,24 @@ constexpr int LDB = 17;  // row stride (words) of the multiplier buffer: conflic
 __host__ __device__ constexpr int mic_of(int W) { return 512 / (64 * W) > 1 ? 512 / (64 * W) : 1; }
 __host__ __device__ constexpr int smax_of(int NTI, int W) { return (NTI * (NTI + 1) / 2 + W - 1) / W; }
 __host__ __device__ inline int ntiles(int m) { return (m + 15) / 16; }
+// Row stride of G (workspace copy and LDS chunks): 16 * nti rounded to an odd multiple of 16, so
+// that the four rows read by one matrix-core operand load fall in disjoint LDS banks.
+__host__ __device__ inline int g_stride(int m) { return 16 * (ntiles(m) | 1); }
 
 struct GLds {
-  int off_L, off_A, off_B, off_pv, off_vb, off_vc, off_d0, off_red, bytes;
+  int off_L, off_A, off_B, off_pv, off_vb, off_vc, off_d0, off_red, off_part, bytes;
 };
 
 // LDS carve-up for one problem (float64 words): the packed factor, which also holds the two G
 // chunks (storage type) during G^T D G; the panel extraction buffer, aliased by the augmented
 // vectors of the solves (x and dx in the backward); the multipliers of one tile column and its
-// pivots; two m_ineq-vectors; the reference diagonal; reduction scratch.
+// pivots; two m_ineq-vectors; the reference diagonal; reduction scratch; partial sums of the
+// column products.
 __host__ __device__ inline GLds group_lds(int m, int mi, int NT, int ts) {
-  const int rows = 16 * ntiles(m);
+  const int rows = 16 * ntiles(m), ldg = g_stride(m);
   const int fac = m * (m + 1) / 2;
-  const int stage = (2 * KC * rows * ts + 7) / 8;
-  const int vec = 2 * (NT > rows ? NT : rows);
+  const int stage = (2 * KC * ldg * ts + 7) / 8;
+  const int vec = 2 * (NT > ldg ? NT : ldg);
   GLds w;
   int o = 0;
   w.off_L = o; o += (fac > stage ? fac : stage);
@@ -65,6 +69,7 @@ __host__ __device__ inline GLds group_lds(int m, int mi, int NT, int ts) {
   w.off_vc = o; o += (mi + 1) & ~1;
   w.off_d0 = o; o += rows;
   w.off_red = o; o += 16;
+  w.off_part = o; o += NT;
   w.bytes = 8 * o;
   return w;
 }
@@ -72,8 +77,8 @@ __host__ __device__ inline GLds group_lds(int m, int mi, int NT, int ts) {
 // Global workspace words of one problem: K's constant part (m x m) and the padded copy of G
 // (m_ineq x 16 ceil(m/16), counted as float64 words whatever the storage type).
 __host__ __device__ inline int64_t group_words(int n, int me, int mi) {
-  const int64_t m = wave::padded_n(n) + me, rows = 16 * ntiles((int)m);
-  return round_up(m * m, 64) + round_up((int64_t)mi * rows, 64);
+  const int64_t m = wave::padded_n(n) + me, ldg = g_stride((int)m);
+  return round_up(m * m, 64) + round_up((int64_t)mi * ldg, 64);
 }
 
 template <int MIC>
@@ -106,6 +111,7 @@ struct GCtx {
   double* vc;
   double* d0;
   double* red;
+  double* part;
 };
 
 template <int W, class Op>
@@ -135,7 +141,7 @@ __device__ __forceinline__ GCtx<TS, NTI, W> make_gctx(int n, int me, int mi, dou
   constexpr int NT = 64 * W, TT = NTI * (NTI + 1) / 2, SMAX = smax_of(NTI, W);
   GCtx<TS, NTI, W> c;
   c.n = n; c.np = wave::padded_n(n); c.me = me; c.mi = mi; c.m = c.np + me;
-  c.nti = ntiles(c.m); c.ldg = 16 * c.nti;
+  c.nti = ntiles(c.m); c.ldg = g_stride(c.m);
   c.t = threadIdx.x;
   c.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   c.l = threadIdx.x & 63; c.rq = c.l >> 4; c.cq = c.l & 15;
@@ -156,7 +162,7 @@ __device__ __forceinline__ GCtx<TS, NTI, W> make_gctx(int n, int me, int mi, dou
   c.pbA = base + L.off_A; c.col = c.pbA;
   c.pbB = base + L.off_B; c.pv = base + L.off_pv;
   c.vb = base + L.off_vb; c.vc = base + L.off_vc;
-  c.d0 = base + L.off_d0; c.red = base + L.off_red;
+  c.d0 = base + L.off_d0; c.red = base + L.off_red; c.part = base + L.off_part;
   c.KQ = ws;
   c.Gw = reinterpret_cast<TS*>(ws + round_up((int64_t)c.m * c.m, 64));
   return c;
@@ -209,42 +215,68 @@ __device__ __forceinline__ void load(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& 
   __syncthreads();
 }
 
-// out[k] = sum_j G[k][j] * col[j] for every row k < mi (col zero beyond m); 16 lanes per row.
+// out[k] = sum_j G[k][j] * col[j] for every row k < mi (col zero beyond m); 16 lanes per row,
+// two rows per lane group per trip so that twice the loads are in flight.
 template <typename TS, int NTI, int W>
 __device__ __forceinline__ void g_rows(const GCtx<TS, NTI, W>& c, double* out) {
   const int mi = c.mi, ldg = c.ldg, nti = c.nti, rq = c.rq, cq = c.cq;
-  for (int kb = 4 * c.w; kb < mi; kb += 4 * W) {
-    const int k = kb + rq;
-    const int kr = k < mi ? k : mi - 1;
-    const TS* g = c.Gw + (int64_t)kr * ldg + cq;
+  for (int kb = 4 * c.w; kb < mi; kb += 8 * W) {
+    const int k0 = kb + rq, k1 = kb + 4 * W + rq;
+    const TS* g0 = c.Gw + (int64_t)(k0 < mi ? k0 : mi - 1) * ldg + cq;
+    const TS* g1 = c.Gw + (int64_t)(k1 < mi ? k1 : mi - 1) * ldg + cq;
     double a0 = 0.0, a1 = 0.0;
-    int u = 0;
-    for (; u + 2 <= nti; u += 2) {
-      a0 += (double)g[16 * u] * c.col[16 * u + cq];
-      a1 += (double)g[16 * u + 16] * c.col[16 * u + 16 + cq];
+#pragma unroll
+    for (int u = 0; u < NTI; ++u) {
+      if (u < nti) {
+        const double xv = c.col[16 * u + cq];
+        a0 += (double)g0[16 * u] * xv;
+        a1 += (double)g1[16 * u] * xv;
+      }
     }
-    if (u < nti) a0 += (double)g[16 * u] * c.col[16 * u + cq];
-    const double v = row16_sum(a0 + a1);
-    if (cq == 0 && k < mi) out[k] = v;
+    a0 = row16_sum(a0);
+    a1 = row16_sum(a1);
+    if (cq == 0 && k0 < mi) out[k0] = a0;
+    if (cq == 0 && k1 < mi) out[k1] = a1;
   }
 }
 
-// sum_k G[k][t] * v[k] over k < mi for this thread's column t < n.
+// Column products out_t = sum_r M[r][t] * v[r] over rows r < R, for every column t < C (M row-major
+// with row stride ld, in global memory), returned to thread t (0 for t >= C).  Columns go to
+// groups of 64 lanes, the rows are split over the NT / ceil64(C) thread groups so that every
+// thread streams a share of M (coalesced across a wavefront); the partial sums meet in LDS in a
+// fixed order.
+template <int W, typename TM>
+__device__ __forceinline__ double col_dot(const TM* M, int64_t ld, int R, int C, const double* v, double* part) {
+  constexpr int NT = 64 * W;
+  const int t = threadIdx.x;
+  const int cp = (C + 63) & ~63;
+  const int P = NT / cp > 0 ? NT / cp : 1;
+  const int j = t % cp, pt = t / cp;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (pt < P && j < C) {
+    const TM* g = M + j;
+    int r = pt;
+    for (; r + 3 * P < R; r += 4 * P) {
+      a0 += (double)g[(int64_t)r * ld] * v[r];
+      a1 += (double)g[(int64_t)(r + P) * ld] * v[r + P];
+      a2 += (double)g[(int64_t)(r + 2 * P) * ld] * v[r + 2 * P];
+      a3 += (double)g[(int64_t)(r + 3 * P) * ld] * v[r + 3 * P];
+    }
+    for (; r < R; r += P) a0 += (double)g[(int64_t)r * ld] * v[r];
+  }
+  __syncthreads();
+  if (pt < P) part[pt * cp + j] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  double out = 0.0;
+  if (t < C) {
+    for (int q = 0; q < P; ++q) out += part[q * cp + t];
+  }
+  return out;
+}
+
 template <typename TS, int NTI, int W>
 __device__ __forceinline__ double g_col(const GCtx<TS, NTI, W>& c, const double* v) {
-  const int mi = c.mi, ldg = c.ldg, t = c.t;
-  if (t >= c.n) return 0.0;
-  const TS* g = c.Gw + t;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int k = 0;
-  for (; k + 4 <= mi; k += 4) {
-    a0 += (double)g[(int64_t)k * ldg] * v[k];
-    a1 += (double)g[(int64_t)(k + 1) * ldg] * v[k + 1];
-    a2 += (double)g[(int64_t)(k + 2) * ldg] * v[k + 2];
-    a3 += (double)g[(int64_t)(k + 3) * ldg] * v[k + 3];
-  }
-  for (; k < mi; ++k) a0 += (double)g[(int64_t)k * ldg] * v[k];
-  return (a0 + a1) + (a2 + a3);
+  return col_dot<W>(c.Gw, c.ldg, c.mi, c.n, v, c.part);
 }
 
 // Residuals of the (relaxed) KKT system (qpax/pdip.py:93-100); returns the infinity norm.
@@ -259,18 +291,8 @@ __device__ __forceinline__ double residuals(const GCtx<TS, NTI, W>& c, GSt<mic_o
     if (k < mi) c.vb[k] = st.z[cc];
   }
   __syncthreads();
-  double acc = st.qa;
-  if (t < m) {
-    const double* kcol = c.KQ + t;
-    double a1 = 0.0;
-    int j = 0;
-    for (; j + 2 <= m; j += 2) {
-      acc += kcol[(int64_t)j * m] * c.col[j];
-      a1 += kcol[(int64_t)(j + 1) * m] * c.col[j + 1];
-    }
-    if (j < m) acc += kcol[(int64_t)j * m] * c.col[j];
-    acc += a1;
-  }
+  // K's constant part times [x; y] (KQ[j m + i] = K[i][j]), then G^T z
+  double acc = st.qa + col_dot<W>(c.KQ, m, m, m, c.col, c.part);
   acc += g_col(c, c.vb);
   st.ra = t < m ? acc : 0.0;
   g_rows(c, c.vc);
@@ -303,7 +325,7 @@ __device__ __forceinline__ double residuals(const GCtx<TS, NTI, W>& c, GSt<mic_o
 template <typename TS, int NTI, int W>
 __device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, bool unit_d) {
   constexpr int NT = 64 * W, MIC = mic_of(W), SMAX = smax_of(NTI, W);
-  const int np = c.np, mi = c.mi, m = c.m, t = c.t, ldg = c.ldg, rows = c.ldg, nti = c.nti;
+  const int np = c.np, mi = c.mi, m = c.m, t = c.t, ldg = c.ldg, rows = 16 * c.nti, nti = c.nti;
   const int rq = c.rq, cq = c.cq;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -327,7 +349,7 @@ __device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>
   // of the next 4 rows load while the current MFMAs run.  Invalid slots read tile (0, 0) and
   // accumulate into their own unused registers.
   {
-    constexpr int EPT = (KC * 16 * NTI + NT - 1) / NT;
+    constexpr int EPT = (KC * 16 * (NTI | 1) + NT - 1) / NT;
     const int csz = KC * ldg;
     const int64_t total = (int64_t)mi * ldg;
     const int nch = (mi + KC - 1) / KC;
@@ -542,26 +564,44 @@ __device__ __forceinline__ void solve(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>&
     if (j0 >= m) break;
     const int jn = m - j0 < 64 ? m - j0 : 64;
     if (w == bw) {
-      for (int j = 0; j < jn; ++j) {
-        const int J = j0 + j;
-        const double lv = Lrow[J < lr ? J : 0];
-        pa -= (t > J ? lv : 0.0) * rl(pa, j);
+      // Lanes of rows >= m hold 0 throughout.  The factor entries load 16 at a time ahead of
+      // the chain; reads past the end of a row stay inside this problem's LDS image and are
+      // discarded by the select.
+      double lv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lv[u] = Lrow[j0 + u];
+#pragma unroll
+      for (int jb = 0; jb < 64; jb += 16) {
+        double nv[16];
+        if (jb + 16 < 64) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) nv[u] = Lrow[j0 + jb + 16 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int J = j0 + jb + u;
+          pa -= (t > J && J < m && t < m ? lv[u] : 0.0) * rl(pa, jb + u);
+        }
+        if (jb + 16 < 64) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) lv[u] = nv[u];
+        }
       }
       if (t < m) yb[t] = pa;
     }
     __syncthreads();
     if (w > bw && t < m) {
-      double a0 = 0.0, a1 = 0.0;
+      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       int j = 0;
-      for (; j + 2 <= jn; j += 2) {
-        a0 += Lrow[j0 + j] * yb[j0 + j];
-        a1 += Lrow[j0 + j + 1] * yb[j0 + j + 1];
+      for (; j + 8 <= jn; j += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += Lrow[j0 + j + u] * yb[j0 + j + u];
       }
-      if (j < jn) a0 += Lrow[j0 + j] * yb[j0 + j];
-      pa -= a0 + a1;
+      for (; j < jn; ++j) a[0] += Lrow[j0 + j] * yb[j0 + j];
+      pa -= ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
   }
-  pa *= st.rdpiv;
+  pa = t < m ? pa * st.rdpiv : 0.0;
   // backward: L^T x = y, blocks from the last
   double* xb = c.col + NT;
   for (int bw = W - 1; bw >= 0; --bw) {
@@ -569,28 +609,53 @@ __device__ __forceinline__ void solve(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>&
     if (j0 >= m) continue;
     const int jn = m - j0 < 64 ? m - j0 : 64;
     if (w == bw) {
-      for (int j = jn - 1; j > 0; --j) {
-        const int J = j0 + j;
-        const bool use = t < J && t >= j0;
-        const double lv = c.Lp[J * (J + 1) / 2 + (use ? t : 0)];
-        pa -= (use ? lv : 0.0) * rl(pa, j);
+      // row J of the factor, entry t (rows clamped to m - 1; reads past the row end stay inside
+      // this problem's LDS image and are discarded by the select)
+      double lv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int J = j0 + 63 - u < m ? j0 + 63 - u : m - 1;
+        lv[u] = c.Lp[J * (J + 1) / 2 + t];
+      }
+#pragma unroll
+      for (int jb = 0; jb < 64; jb += 16) {
+        double nv[16];
+        if (jb + 16 < 64) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int J = j0 + 63 - (jb + 16 + u) < m ? j0 + 63 - (jb + 16 + u) : m - 1;
+            nv[u] = c.Lp[J * (J + 1) / 2 + t];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = 63 - (jb + u);
+          const int J = j0 + j;
+          if (j > 0) pa -= (t < J && J < m ? lv[u] : 0.0) * rl(pa, j);
+        }
+        if (jb + 16 < 64) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) lv[u] = nv[u];
+        }
       }
       if (t < m) xb[t] = pa;
     }
     __syncthreads();
     if (w < bw) {
-      double a0 = 0.0, a1 = 0.0;
+      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       int j = 0;
-      for (; j + 2 <= jn; j += 2) {
-        const int J = j0 + j;
-        a0 += c.Lp[J * (J + 1) / 2 + t] * xb[J];
-        a1 += c.Lp[(J + 1) * (J + 2) / 2 + t] * xb[J + 1];
+      for (; j + 8 <= jn; j += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int J = j0 + j + u;
+          a[u] += c.Lp[J * (J + 1) / 2 + t] * xb[J];
+        }
       }
-      if (j < jn) {
+      for (; j < jn; ++j) {
         const int J = j0 + j;
-        a0 += c.Lp[J * (J + 1) / 2 + t] * xb[J];
+        a[0] += c.Lp[J * (J + 1) / 2 + t] * xb[J];
       }
-      pa -= a0 + a1;
+      pa -= ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
   }
   (void)l;
