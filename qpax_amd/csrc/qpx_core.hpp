@@ -190,18 +190,98 @@ __device__ void block_bsub_t(const double* L, const double* rcp, int m, double* 
 // --------------------------------------------------------------------------------------------
 enum DMode : int { kDzs = 0, kDunit = 1, kDgiven = 2 };
 
+// Matrix-core G^T D G for n >= kMfmaMinN: the lower triangle of H = Q + G^T D G as 16x16 tiles
+// accumulated with v_mfma_f64_16x16x4f64.  Wavefront w takes tiles w, w + W, ... (row-major over
+// the lower triangle) in batches of kGtdgBatch accumulators; each batch streams G through the
+// dynamic LDS in chunks of kGtdgRows rows (row stride 16 * (tiles | 1), zero-padded columns; the
+// odd tile count keeps an operand load's four rows in disjoint banks) with D beside them.
+constexpr int kMfmaMinN = 64;
+constexpr int kMfmaMaxN = 272;
+constexpr int kGtdgRows = 16;
+constexpr int kGtdgBatch = 8;
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int gtdg_stride(int n) { return 16 * (((n + 15) / 16) | 1); }
+// Dynamic LDS bytes the generic kernels need for a problem with n variables.
+__host__ __device__ inline int gtdg_lds_bytes(int n) {
+  return (n >= kMfmaMinN && n <= kMfmaMaxN) ? 8 * (kGtdgRows * gtdg_stride(n) + kGtdgRows) : 0;
+}
+
+template <int NT>
+__device__ void gtdg_mfma(const View& v, double* chunk) {
+  constexpr int W = NT / 64;
+  const int n = v.n, mi = v.mi, tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const int nt = (n + 15) / 16, ldl = gtdg_stride(n), T = nt * (nt + 1) / 2;
+  double* cD = chunk + kGtdgRows * ldl;
+  for (int base = 0; base < T; base += W * kGtdgBatch) {
+    int tI[kGtdgBatch], tJ[kGtdgBatch];
+    bool ok[kGtdgBatch];
+#pragma unroll
+    for (int b = 0; b < kGtdgBatch; ++b) {
+      const int idx = base + b * W + w;
+      int I = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+      while ((I + 1) * (I + 2) / 2 <= idx) ++I;
+      while (I * (I + 1) / 2 > idx) --I;
+      ok[b] = idx < T;
+      tI[b] = ok[b] ? I : 0;
+      tJ[b] = ok[b] ? idx - I * (I + 1) / 2 : 0;
+    }
+    d4v acc[kGtdgBatch];
+#pragma unroll
+    for (int b = 0; b < kGtdgBatch; ++b) acc[b] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
+      __syncthreads();
+      for (int r = 0; r < kGtdgRows; ++r) {
+        const int k = k0 + r;
+        for (int c = tid; c < ldl; c += NT) chunk[r * ldl + c] = (k < mi && c < n) ? v.G[(int64_t)k * n + c] : 0.0;
+      }
+      for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? v.D[k0 + r] : 0.0;
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
+        const double* gr = chunk + (4 * ks + rq) * ldl + cq;
+        const double dk = cD[4 * ks + rq];
+        double a[kGtdgBatch], bb[kGtdgBatch];
+#pragma unroll
+        for (int b = 0; b < kGtdgBatch; ++b) {
+          a[b] = gr[16 * tI[b]] * dk;
+          bb[b] = gr[16 * tJ[b]];
+        }
+#pragma unroll
+        for (int b = 0; b < kGtdgBatch; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < kGtdgBatch; ++b) {
+      if (!ok[b]) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * tI[b] + 4 * r + rq, j = 16 * tJ[b] + cq;
+        if (i < n && j <= i) v.H[i * n + j] = v.Q[i * n + j] + acc[b][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <int NT>
 __device__ void stage_factor(const View& v, int dmode) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   if (dmode != kDgiven)
     for (int k = tid; k < mi; k += NT) v.D[k] = dmode == kDunit ? 1.0 : v.z[k] / v.s[k];
   __syncthreads();
-  for (int e = tid; e < n * n; e += NT) {
-    const int i = e / n, j = e - i * n;
-    if (j > i) continue;
-    double acc = 0.0;
-    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * (v.D[k] * v.G[k * n + j]);
-    v.H[e] = v.Q[e] + acc;
+  if (n >= kMfmaMinN && n <= kMfmaMaxN) {
+    extern __shared__ __align__(16) unsigned char qpx_smem[];
+    gtdg_mfma<NT>(v, reinterpret_cast<double*>(qpx_smem));
+  } else {
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e - i * n;
+      if (j > i) continue;
+      double acc = 0.0;
+      for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * (v.D[k] * v.G[k * n + j]);
+      v.H[e] = v.Q[e] + acc;
+    }
   }
   __syncthreads();
   block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);

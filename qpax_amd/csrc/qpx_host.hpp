@@ -50,11 +50,11 @@ int group_lds_bytes(int n, int me, int mi, int ts, GroupVariant v);
 
 // Launchers of the workgroup-per-QP kernels (any size), NT in {64, 128, 256}.
 template <int NT, typename TS>
-hipError_t launch_generic_solve(const Args<TS>& a, unsigned grid, hipStream_t st, int relax);
+hipError_t launch_generic_solve(const Args<TS>& a, unsigned grid, int lds, hipStream_t st, int relax);
 template <int NT, typename TS>
-hipError_t launch_generic_bwd(const Args<TS>& a, unsigned grid, hipStream_t st);
+hipError_t launch_generic_bwd(const Args<TS>& a, unsigned grid, int lds, hipStream_t st);
 template <int NT, typename TS>
-int generic_occupancy();  // resident workgroups per CU
+int generic_occupancy(int lds);  // resident workgroups per CU
 
 namespace group {
 template <typename TS, int NTI, int W>

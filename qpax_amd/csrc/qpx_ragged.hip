@@ -19,6 +19,7 @@
 #include "../../include/qpx.h"
 #include "qpx_args.hpp"
 #include "qpx_host.hpp"
+#include "qpx_core.hpp"
 #include "qpx_wave_launch.hpp"
 
 namespace {
@@ -104,8 +105,9 @@ unsigned grid_of(const Bucket& b) {
   } else if (b.kind == kGroup) {
     per_cu = qpx::group::occupancy<TS>(b.gv, b.lds_per_wave);
   } else {
-    per_cu = b.nt == 64 ? qpx::generic_occupancy<64, TS>() : b.nt == 128 ? qpx::generic_occupancy<128, TS>()
-                                                                          : qpx::generic_occupancy<256, TS>();
+    per_cu = b.nt == 64    ? qpx::generic_occupancy<64, TS>(b.lds_per_wave)
+             : b.nt == 128 ? qpx::generic_occupancy<128, TS>(b.lds_per_wave)
+                           : qpx::generic_occupancy<256, TS>(b.lds_per_wave);
   }
   if (per_cu < 1) per_cu = 1;
   const int64_t slots = (int64_t)per_cu * qpx::host::cu_count();
@@ -119,9 +121,9 @@ hipError_t launch_bucket_fwd(const qpx::Args<TS>& a, const Bucket& b, hipStream_
   const unsigned grid = grid_of<TS>(b);
   if (b.kind == kWave) return qpx::wave::launch_solve(b.wv, a, grid, b.wpb, st, relax);
   if (b.kind == kGroup) return qpx::group::launch_solve(b.gv, a, grid, b.lds_per_wave, st, relax);
-  if (b.nt == 64) return qpx::launch_generic_solve<64, TS>(a, grid, st, relax);
-  if (b.nt == 128) return qpx::launch_generic_solve<128, TS>(a, grid, st, relax);
-  return qpx::launch_generic_solve<256, TS>(a, grid, st, relax);
+  if (b.nt == 64) return qpx::launch_generic_solve<64, TS>(a, grid, b.lds_per_wave, st, relax);
+  if (b.nt == 128) return qpx::launch_generic_solve<128, TS>(a, grid, b.lds_per_wave, st, relax);
+  return qpx::launch_generic_solve<256, TS>(a, grid, b.lds_per_wave, st, relax);
 }
 
 template <typename TS>
@@ -129,9 +131,9 @@ hipError_t launch_bucket_bwd(const qpx::Args<TS>& a, const Bucket& b, hipStream_
   const unsigned grid = grid_of<TS>(b);
   if (b.kind == kWave) return qpx::wave::launch_bwd(b.wv, a, grid, b.wpb, st);
   if (b.kind == kGroup) return qpx::group::launch_bwd(b.gv, a, grid, b.lds_per_wave, st);
-  if (b.nt == 64) return qpx::launch_generic_bwd<64, TS>(a, grid, st);
-  if (b.nt == 128) return qpx::launch_generic_bwd<128, TS>(a, grid, st);
-  return qpx::launch_generic_bwd<256, TS>(a, grid, st);
+  if (b.nt == 64) return qpx::launch_generic_bwd<64, TS>(a, grid, b.lds_per_wave, st);
+  if (b.nt == 128) return qpx::launch_generic_bwd<128, TS>(a, grid, b.lds_per_wave, st);
+  return qpx::launch_generic_bwd<256, TS>(a, grid, b.lds_per_wave, st);
 }
 
 struct FwdIO {
@@ -315,6 +317,7 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
       total += cost[k];
       if (bks[j].kind == kWave) lds = std::max(lds, qpx::wave::lds_layout(e.n, e.me, e.mi, ts).bytes);
       if (bks[j].kind == kGroup) lds = std::max(lds, qpx::host::group_lds_bytes(e.n, e.me, e.mi, ts, bks[j].gv));
+      if (bks[j].kind == kBlock) lds = std::max(lds, qpx::gtdg_lds_bytes(e.n));
     }
     std::vector<size_t> idx(mem.size());
     std::iota(idx.begin(), idx.end(), 0);

@@ -410,30 +410,30 @@ namespace qpx {
 
 // Launchers of the workgroup-per-QP fused kernels, shared with the ragged-batch scheduler.
 template <int NT, typename TS>
-hipError_t launch_generic_solve(const Args<TS>& a, unsigned grid, hipStream_t st, int relax) {
-  hipLaunchKernelGGL((k_fused_solve<NT, TS>), dim3(grid), dim3(NT), 0, st, a, relax);
+hipError_t launch_generic_solve(const Args<TS>& a, unsigned grid, int lds, hipStream_t st, int relax) {
+  hipLaunchKernelGGL((k_fused_solve<NT, TS>), dim3(grid), dim3(NT), lds, st, a, relax);
   return hipGetLastError();
 }
 
 template <int NT, typename TS>
-hipError_t launch_generic_bwd(const Args<TS>& a, unsigned grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_fused_bwd<NT, TS>), dim3(grid), dim3(NT), 0, st, a);
+hipError_t launch_generic_bwd(const Args<TS>& a, unsigned grid, int lds, hipStream_t st) {
+  hipLaunchKernelGGL((k_fused_bwd<NT, TS>), dim3(grid), dim3(NT), lds, st, a);
   return hipGetLastError();
 }
 
 template <int NT, typename TS>
-int generic_occupancy() {
+int generic_occupancy(int lds) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_fused_solve<NT, TS>), NT, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_fused_solve<NT, TS>), NT,
+                                                   (size_t)lds) != hipSuccess)
     nb = 0;
   return nb;
 }
 
-#define QPX_GEN_INST(NT, TS)                                                                   \
-  template hipError_t launch_generic_solve<NT, TS>(const Args<TS>&, unsigned, hipStream_t, int); \
-  template hipError_t launch_generic_bwd<NT, TS>(const Args<TS>&, unsigned, hipStream_t);        \
-  template int generic_occupancy<NT, TS>();
+#define QPX_GEN_INST(NT, TS)                                                                        \
+  template hipError_t launch_generic_solve<NT, TS>(const Args<TS>&, unsigned, int, hipStream_t, int); \
+  template hipError_t launch_generic_bwd<NT, TS>(const Args<TS>&, unsigned, int, hipStream_t);        \
+  template int generic_occupancy<NT, TS>(int);
 QPX_GEN_INST(64, float)
 QPX_GEN_INST(128, float)
 QPX_GEN_INST(256, float)
@@ -572,6 +572,7 @@ bool wave_plan(qpx::Args<TS>& a, qpx::host::WaveVariant* v) {
 
 template <int NT, typename TS>
 qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
+  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -586,32 +587,32 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
       QPX_HIP(qpx::group::launch_solve(gv, a, (unsigned)a.B, lds, st, relax ? 1 : 0));
       return QPX_OK;
     }
-    hipLaunchKernelGGL((qpx::k_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
+    hipLaunchKernelGGL((qpx::k_fused_solve<NT, TS>), grid, block, lds, st, a, relax ? 1 : 0);
     QPX_HIP(hipGetLastError());
     return QPX_OK;
   }
   int* host = pinned_counter();
   if (!host) return fail(QPX_ERR_HIP, "hipHostMalloc of the plain-mode counter failed");
-  hipLaunchKernelGGL((qpx::k_setup<NT, TS>), grid, block, 0, st, a);
+  hipLaunchKernelGGL((qpx::k_setup<NT, TS>), grid, block, lds, st, a);
   QPX_HIP(hipGetLastError());
   for (int phase = 0; phase < (relax ? 2 : 1); ++phase) {
     const int rl = phase;
     for (int it = 0; it < qpx::kMaxIters; ++it) {
-      hipLaunchKernelGGL((qpx::k_resid<NT, TS>), grid, block, 0, st, a, rl);
-      hipLaunchKernelGGL((qpx::k_factor<NT, TS>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((qpx::k_resid<NT, TS>), grid, block, lds, st, a, rl);
+      hipLaunchKernelGGL((qpx::k_factor<NT, TS>), grid, block, lds, st, a);
       if (!rl) {
-        hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, 0, st, a, 1);
-        hipLaunchKernelGGL((qpx::k_center<NT, TS>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, lds, st, a, 1);
+        hipLaunchKernelGGL((qpx::k_center<NT, TS>), grid, block, lds, st, a);
       }
-      hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, 0, st, a, 0);
-      hipLaunchKernelGGL((qpx::k_step<NT, TS>), grid, block, 0, st, a, rl);
+      hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, lds, st, a, 0);
+      hipLaunchKernelGGL((qpx::k_step<NT, TS>), grid, block, lds, st, a, rl);
       QPX_HIP(hipGetLastError());
       QPX_HIP(hipMemcpyAsync(host, a.counter, sizeof(int), hipMemcpyDeviceToHost, st));
       QPX_HIP(hipStreamSynchronize(st));
       if (*host == 0) break;
     }
-    if (!rl) hipLaunchKernelGGL((qpx::k_finish_solve<NT, TS>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((qpx::k_finish_relax<NT, TS>), grid, block, 0, st, a);
+    if (!rl) hipLaunchKernelGGL((qpx::k_finish_solve<NT, TS>), grid, block, lds, st, a);
+    else hipLaunchKernelGGL((qpx::k_finish_relax<NT, TS>), grid, block, lds, st, a);
     QPX_HIP(hipGetLastError());
   }
   return QPX_OK;
@@ -619,6 +620,7 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
 
 template <int NT, typename TS>
 qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
+  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -633,11 +635,11 @@ qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
       QPX_HIP(qpx::group::launch_bwd(gv, a, (unsigned)a.B, lds, st));
       return QPX_OK;
     }
-    hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, lds, st, a);
   } else {
-    hipLaunchKernelGGL((qpx::k_bwd_load<NT, TS>), grid, block, 0, st, a);
-    hipLaunchKernelGGL((qpx::k_bwd_factor<NT, TS>), grid, block, 0, st, a);
-    hipLaunchKernelGGL((qpx::k_bwd_solve<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_bwd_load<NT, TS>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((qpx::k_bwd_factor<NT, TS>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((qpx::k_bwd_solve<NT, TS>), grid, block, lds, st, a);
   }
   QPX_HIP(hipGetLastError());
   return QPX_OK;
@@ -685,35 +687,36 @@ qpx_status backward_impl(int mode, int64_t B, int n, int me, int mi, const void*
 // ------------------------------------ elastic -----------------------------------------------
 template <int NT, typename TS>
 qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
+  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
-    hipLaunchKernelGGL((qpx::k_e_fused_solve<NT, TS>), grid, block, 0, st, a, relax ? 1 : 0);
+    hipLaunchKernelGGL((qpx::k_e_fused_solve<NT, TS>), grid, block, lds, st, a, relax ? 1 : 0);
     QPX_HIP(hipGetLastError());
     return QPX_OK;
   }
   int* host = pinned_counter();
   if (!host) return fail(QPX_ERR_HIP, "hipHostMalloc of the plain-mode counter failed");
-  hipLaunchKernelGGL((qpx::k_e_setup<NT, TS>), grid, block, 0, st, a);
+  hipLaunchKernelGGL((qpx::k_e_setup<NT, TS>), grid, block, lds, st, a);
   QPX_HIP(hipGetLastError());
   for (int phase = 0; phase < (relax ? 2 : 1); ++phase) {
     const int rl = phase;
     for (int it = 0; it < qpx::kMaxIters; ++it) {
-      hipLaunchKernelGGL((qpx::k_e_resid<NT, TS>), grid, block, 0, st, a, rl);
-      hipLaunchKernelGGL((qpx::k_e_factor<NT, TS>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((qpx::k_e_resid<NT, TS>), grid, block, lds, st, a, rl);
+      hipLaunchKernelGGL((qpx::k_e_factor<NT, TS>), grid, block, lds, st, a);
       if (!rl) {
-        hipLaunchKernelGGL((qpx::k_e_solve<NT, TS>), grid, block, 0, st, a, 1);
-        hipLaunchKernelGGL((qpx::k_e_center<NT, TS>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((qpx::k_e_solve<NT, TS>), grid, block, lds, st, a, 1);
+        hipLaunchKernelGGL((qpx::k_e_center<NT, TS>), grid, block, lds, st, a);
       }
-      hipLaunchKernelGGL((qpx::k_e_solve<NT, TS>), grid, block, 0, st, a, 0);
-      hipLaunchKernelGGL((qpx::k_e_step<NT, TS>), grid, block, 0, st, a, rl);
+      hipLaunchKernelGGL((qpx::k_e_solve<NT, TS>), grid, block, lds, st, a, 0);
+      hipLaunchKernelGGL((qpx::k_e_step<NT, TS>), grid, block, lds, st, a, rl);
       QPX_HIP(hipGetLastError());
       QPX_HIP(hipMemcpyAsync(host, a.counter, sizeof(int), hipMemcpyDeviceToHost, st));
       QPX_HIP(hipStreamSynchronize(st));
       if (*host == 0) break;
     }
-    if (!rl) hipLaunchKernelGGL((qpx::k_e_finish_solve<NT, TS>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((qpx::k_e_finish_relax<NT, TS>), grid, block, 0, st, a);
+    if (!rl) hipLaunchKernelGGL((qpx::k_e_finish_solve<NT, TS>), grid, block, lds, st, a);
+    else hipLaunchKernelGGL((qpx::k_e_finish_relax<NT, TS>), grid, block, lds, st, a);
     QPX_HIP(hipGetLastError());
   }
   return QPX_OK;
@@ -721,14 +724,15 @@ qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st)
 
 template <int NT, typename TS>
 qpx_status run_e_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
+  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
-    hipLaunchKernelGGL((qpx::k_e_fused_bwd<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_e_fused_bwd<NT, TS>), grid, block, lds, st, a);
   } else {
-    hipLaunchKernelGGL((qpx::k_e_bwd_load<NT, TS>), grid, block, 0, st, a);
-    hipLaunchKernelGGL((qpx::k_e_bwd_factor<NT, TS>), grid, block, 0, st, a);
-    hipLaunchKernelGGL((qpx::k_e_bwd_solve<NT, TS>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((qpx::k_e_bwd_load<NT, TS>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((qpx::k_e_bwd_factor<NT, TS>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((qpx::k_e_bwd_solve<NT, TS>), grid, block, lds, st, a);
   }
   QPX_HIP(hipGetLastError());
   return QPX_OK;

@@ -105,11 +105,14 @@ def uneven(B, seed=4, n_lo=8, n_hi=128):
     return out
 
 
-def elastic_infeasible(B, n, mi, frac=0.2, seed=5):
-    """Random feasible QPs without equality rows where a fraction ``frac`` of problems get
-    ``h -= 3|N(0,1)|``, which typically makes them infeasible.  Returns (Q, q, G, h, shifted)."""
-    Q, q, _, _, G, h = random_feasible(B, n, 0, mi, seed=seed)
+def elastic_infeasible(B, n, mi, frac=0.2, seed=5, me=0):
+    """Random feasible QPs (``me`` equality rows) where a fraction ``frac`` of problems get
+    ``h -= 3|N(0,1)|``, which typically makes them infeasible.  Returns (Q, q, G, h, shifted), or
+    (Q, q, A, b, G, h, shifted) when ``me`` > 0."""
+    Q, q, A, b, G, h = random_feasible(B, n, me, mi, seed=seed)
     rng = np.random.default_rng(seed + 1000)
     shifted = rng.random(B) < frac
     h = h - np.where(shifted[:, None], 3.0 * np.abs(rng.standard_normal((B, mi))), 0.0)
+    if me:
+        return Q, q, A, b, G, h, shifted
     return Q, q, G, h, shifted

@@ -91,6 +91,37 @@ def test_elastic_grads_match_oracle(relax, mode):
         assert e.max() < tol
 
 
+def test_elastic_c5_size_f32():
+    """The elastic QP at the n=256, m_eq=32, m_ineq=512 size (20% of problems shifted infeasible),
+    float32, forward + backward through the elastic primal on a small batch, against the float64
+    oracle (matrix-core G^T D G in the workgroup kernels)."""
+    import qpax_amd
+
+    Q, q, A, b, G, h, shifted = P.elastic_infeasible(6, 256, 512, frac=0.34, seed=5, me=32)
+    pen = 14.0
+    print("[case] elastic c5 size B=6", flush=True)
+    data = [np.asarray(v, np.float32) for v in (Q, q, A, b, G, h)]
+    ts = [torch.tensor(v, device="cuda", requires_grad=True) for v in data]
+    x = qpax_amd.solve_qp_elastic_primal(ts[0], ts[1], ts[4], ts[5], pen, A=ts[2], b=ts[3], solver_tol=1e-5,
+                                         target_kappa=1e-3)
+    ((x - 1.0) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    d64 = [v.astype(np.float64) for v in data]
+    xo, rel, info = O.solve_qp_elastic_primal_fwd(d64[0], d64[1], d64[4], d64[5], pen, solver_tol=1e-5,
+                                                  target_kappa=1e-3, A=d64[2], b=d64[3])
+    go = O.solve_qp_elastic_primal_bwd(d64[0], d64[1], d64[4], d64[5], rel, 2 * (xo - 1), A=d64[2])
+    ok = info[0] == 1
+    assert ok.mean() > 0.6
+    ex = rel_err(to_np(x).astype(np.float64)[ok], xo[ok])
+    print(f"x err max {ex.max():.2e}")
+    assert ex.max() < 1e-4
+    go = (go[0], go[1], go[4], go[5], go[2], go[3])  # oracle order (Q, q, G, h, A, b)
+    for nm, t, g in zip("Q q A b G h".split(), ts, go):
+        e = rel_err(to_np(t.grad).astype(np.float64)[ok], g[ok])
+        print(f"d{nm}: {e.max():.2e}")
+        assert e.max() < 1e-3
+
+
 def test_elastic_equality_extension():
     import qpax_amd
 
