@@ -213,6 +213,60 @@ def uneven_bench(torch, args):
     print(json.dumps(res), flush=True)
 
 
+def elastic_bench(torch, args):
+    """Elastic variant: 1024 QPs n=256, m_eq=32, m_ineq=512 (seed 5), 20% of problems with
+    h -= 3|N(0,1)|, penalty 14, float32 fwd (solve + relax) + bwd through solve_qp_elastic_primal
+    with loss ||x - 1||^2; one step = forward + backward of the whole batch."""
+    import qpax_amd
+    from qpax_amd import problems as P
+
+    B = args.elastic_batch
+    Q, q, A, b, G, h, shifted = P.elastic_infeasible(B, 256, 512, frac=0.2, seed=5, me=32)
+    data = [np.asarray(v, np.float32) for v in (Q, q, A, b, G, h)]
+    dev = torch.device("cuda", 0)
+    ts = [torch.tensor(v, device=dev, requires_grad=True) for v in data]
+    pen = 14.0
+
+    def step():
+        for t in ts:
+            t.grad = None
+        x = qpax_amd.solve_qp_elastic_primal(ts[0], ts[1], ts[4], ts[5], pen, A=ts[2], b=ts[3], solver_tol=TOL,
+                                             target_kappa=KAPPA)
+        ((x - 1.0) ** 2).sum().backward()
+        return x
+
+    ms = 1e3 * time_steps(torch, step, args.steps, args.warmup)
+    x = step().detach().cpu().numpy().astype(np.float64)
+    res = {
+        "metric": "fwd+bwd QPs/s, elastic QP n=256 m_eq=32 m_ineq=512",
+        "value": round(B / (ms * 1e-3), 2), "unit": "QPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{B} elastic QPs n=256 m_eq=32 m_ineq=512, 20% shifted infeasible (h -= 3|N(0,1)|), "
+                               "penalty 14, seed 5, float32 inputs/outputs, float64 arithmetic, "
+                               "solve_qp_elastic_primal fwd+bwd, tol 1e-5, kappa 1e-3",
+                   "batch": B, "io_dtype": "f32"},
+        "shifted_frac": float(shifted.mean()),
+    }
+    if not args.no_cpu:
+        from oracle import qp_oracle as O
+
+        S = args.elastic_check
+        d64 = [v[:S].astype(np.float64) for v in data]
+        t0 = time.perf_counter()
+        xo, rel, info = O.solve_qp_elastic_primal_fwd(d64[0], d64[1], d64[4], d64[5], pen, solver_tol=TOL,
+                                                      target_kappa=KAPPA, A=d64[2], b=d64[3])
+        cpu_s = time.perf_counter() - t0
+        ok = info[0] == 1
+        err = np.linalg.norm(x[:S] - xo, axis=1) / np.maximum(np.linalg.norm(xo, axis=1), 1e-12)
+        res["x_rel_err_vs_f64_oracle"] = {"max_converged": float(err[ok].max()), "median": float(np.median(err)),
+                                          "checked": S}
+        res["cpu_baseline"] = {"value": round(S / cpu_s, 3), "unit": "QPs/s (forward only)", "cores": 1,
+                               "kind": "port", "sample": f"first {S} problems, forward (solve + relax), float64 "
+                                                         "numpy oracle"}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,8 +277,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only run a few fused steps (for rocprofv3)")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a plain device copy (counter calibration)")
-    ap.add_argument("--workload", choices=("c3", "uneven"), default="c3",
-                    help="c3: the headline n=50 B=4096 workload; uneven: 16384 mixed-size QPs (ragged plan)")
+    ap.add_argument("--workload", choices=("c3", "uneven", "elastic"), default="c3",
+                    help="c3: the headline n=50 B=4096 workload; uneven: 16384 mixed-size QPs (ragged plan); "
+                         "elastic: 1024 elastic QPs n=256 m_eq=32 m_ineq=512")
+    ap.add_argument("--elastic-batch", type=int, default=1024)
+    ap.add_argument("--elastic-check", type=int, default=4, help="problems of the elastic batch checked vs the oracle")
     ap.add_argument("--uneven-batch", type=int, default=16384)
     ap.add_argument("--uneven-check", type=int, default=32, help="problems of the uneven batch checked vs the oracle")
     args = ap.parse_args()
@@ -249,6 +306,9 @@ def main():
 
     if args.workload == "uneven":
         uneven_bench(torch, args)
+        return
+    if args.workload == "elastic":
+        elastic_bench(torch, args)
         return
 
     data, ts = make_inputs(torch, dev)
