@@ -183,18 +183,26 @@ __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     if (k < mi) c.vb[k] = st.z[cc];
   }
   WSYNC();
+  // K's constant part times [x; y]: column j of KQ is read coalesced across the lanes, 16 loads in
+  // flight per trip (rows >= m read row m - 1 and are discarded)
   double acc = st.qa;
-  if (l < m) {
-    const double* kcol = c.KQ + l;
-    double a1 = 0.0;
+  {
+    const double* kcol = c.KQ + (l < m ? l : m - 1);
+    double a[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) a[u] = 0.0;
     int j = 0;
-    for (; j + 2 <= m; j += 2) {
-      const double2 x2 = *reinterpret_cast<const double2*>(c.col + j);
-      acc += kcol[j * m] * x2.x;
-      a1 += kcol[(j + 1) * m] * x2.y;
+    for (; j + 16 <= m; j += 16) {
+      double kv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) kv[u] = kcol[(j + u) * m];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[u] += kv[u] * c.col[j + u];
     }
-    if (j < m) acc += kcol[j * m] * c.col[j];
-    acc += a1;
+    for (; j < m; ++j) a[0] += kcol[j * m] * c.col[j];
+#pragma unroll
+    for (int u = 1; u < 16; ++u) a[0] += a[u];
+    if (l < m) acc += a[0];
   }
   if (l < n) acc += gcol_dot(c.Gs, n, mi, l, c.vb);
   st.ra = l < m ? acc : 0.0;
@@ -395,24 +403,63 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   if (l < n) pa += gcol_dot(c.Gs, n, mi, l, c.vb);
   if (l >= m) pa = 0.0;
   // forward: unit-lower L y = p; lane l reads row l of L (rows >= m read row m-1, unused)
+  // The factor entries load 16 at a time ahead of the dependency chain.  Reads past the end of
+  // a row run into the following rows or the panel buffer behind the factor (inside this
+  // wavefront's LDS image) and are discarded by the select; lanes of rows >= m stay finite and
+  // only ever meet zero coefficients.
+  constexpr int NB = NMAX / 16;
   const int lr = l < m ? l : m - 1;
   const double* Lrow = c.Lp + lr * (lr + 1) / 2;
+  {
+    double lv[16];
 #pragma unroll
-  for (int j = 0; j < NMAX; ++j) {
-    if (j < m) {
-      const double lv = Lrow[j < lr ? j : 0];
-      pa -= (l > j ? lv : 0.0) * rl(pa, j);
+    for (int u = 0; u < 16; ++u) lv[u] = Lrow[u];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      double nv[16];
+      if (b + 1 < NB) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) nv[u] = Lrow[16 * (b + 1) + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = 16 * b + u;
+        pa -= (l > j && j < m ? lv[u] : 0.0) * rl(pa, j);
+      }
+      if (b + 1 < NB) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) lv[u] = nv[u];
+      }
     }
   }
-  pa *= st.rdpiv;
-  // backward: L^T x = y, column j of L^T is row j of L.  For lanes >= j the read runs past row j
-  // into the following rows / the panel buffer that follows the factor (in bounds of this
-  // wavefront's LDS image) and is masked off.
+  pa = l < m ? pa * st.rdpiv : 0.0;
+  // backward: L^T x = y, column j of L^T is row j of L (rows clamped to m - 1 for the address)
+  {
+    double lv[16];
 #pragma unroll
-  for (int j = NMAX - 1; j > 0; --j) {
-    if (j < m) {
-      const double lv = c.Lp[j * (j + 1) / 2 + l];
-      pa -= (l < j ? lv : 0.0) * rl(pa, j);
+    for (int u = 0; u < 16; ++u) {
+      const int j = NMAX - 1 - u, jj = j < m ? j : m - 1;
+      lv[u] = c.Lp[jj * (jj + 1) / 2 + l];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      double nv[16];
+      if (b + 1 < NB) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = NMAX - 1 - (16 * (b + 1) + u), jj = j < m ? j : m - 1;
+          nv[u] = c.Lp[jj * (jj + 1) / 2 + l];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = NMAX - 1 - (16 * b + u);
+        if (j > 0) pa -= (l < j && j < m ? lv[u] : 0.0) * rl(pa, j);
+      }
+      if (b + 1 < NB) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) lv[u] = nv[u];
+      }
     }
   }
   st.da = l < m ? pa : 0.0;
