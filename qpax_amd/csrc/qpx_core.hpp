@@ -158,7 +158,7 @@ __device__ void block_cholesky(double* M, int m, double* diag0, double* rcp) {
 
 // Forward substitution L y = b for one right-hand side; b is overwritten with y.
 template <int NT>
-__device__ void block_fsub(const double* L, const double* rcp, int m, double* b) {
+__device__ void block_fsub_scalar(const double* L, const double* rcp, int m, double* b) {
   const int tid = threadIdx.x;
   for (int j = 0; j < m; ++j) {
     const double yj = b[j] * rcp[j];
@@ -171,7 +171,7 @@ __device__ void block_fsub(const double* L, const double* rcp, int m, double* b)
 
 // Back substitution L^T x = b; b is overwritten with x.
 template <int NT>
-__device__ void block_bsub_t(const double* L, const double* rcp, int m, double* b) {
+__device__ void block_bsub_t_scalar(const double* L, const double* rcp, int m, double* b) {
   const int tid = threadIdx.x;
   for (int j = m - 1; j >= 0; --j) {
     const double xj = b[j] * rcp[j];
@@ -202,9 +202,13 @@ constexpr int kGtdgBatch = 8;
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline int gtdg_stride(int n) { return 16 * (((n + 15) / 16) | 1); }
-// Dynamic LDS bytes the generic kernels need for a problem with n variables.
+constexpr int kPanelLd = 17;  // row stride of the Cholesky panel in LDS
+// Dynamic LDS bytes the generic kernels need for a problem with n variables: the G chunk of the
+// matrix-core G^T D G, and (aliasing it) the 16-column panel of the blocked Cholesky.
 __host__ __device__ inline int gtdg_lds_bytes(int n) {
-  return (n >= kMfmaMinN && n <= kMfmaMaxN) ? 8 * (kGtdgRows * gtdg_stride(n) + kGtdgRows) : 0;
+  if (n < kMfmaMinN || n > kMfmaMaxN) return 0;
+  const int a = kGtdgRows * gtdg_stride(n) + kGtdgRows, b = kPanelLd * 16 * ((n + 15) / 16);
+  return 8 * (a > b ? a : b);
 }
 
 template <int NT>
@@ -265,6 +269,195 @@ __device__ void gtdg_mfma(const View& v, double* chunk) {
   __syncthreads();
 }
 
+// Blocked Cholesky for 64 <= m <= kMfmaMaxN, same contract as block_cholesky.  Block column k
+// (16 columns, rows 16k..m-1) is loaded into the LDS panel, factored there column by column, and
+// written back; the trailing tiles (I, J), k < J <= I, then receive -L_Ik L_Jk^T as four
+// v_mfma_f64_16x16x4f64 each, read-modify-written in place, spread round-robin over the
+// wavefronts.
+template <int NT>
+__device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rcp, double* P) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const int nt = (m + 15) / 16;
+  for (int i = tid; i < m; i += NT) diag0[i] = M[i * m + i];
+  __syncthreads();
+  for (int k = 0; k < nt; ++k) {
+    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16, R = m - c0;
+    for (int e = tid; e < R * 16; e += NT) {
+      const int r = e >> 4, c = e & 15;
+      P[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? M[(c0 + r) * m + c0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < cw; ++j) {
+      const double d = P[j * kPanelLd + j];
+      const bool ok = d > kPivotRel * fabs(diag0[c0 + j]);
+      const double ljj = ok ? sqrt(d) : kHugePivot;
+      const double inv = 1.0 / ljj;
+      for (int r = j + 1 + tid; r < R; r += NT) P[r * kPanelLd + j] *= inv;
+      __syncthreads();
+      if (tid == 0) { P[j * kPanelLd + j] = ljj; rcp[c0 + j] = inv; }
+      for (int e = tid; e < R * 16; e += NT) {
+        const int r = e >> 4, c = e & 15;
+        if (c > j && c < cw && r >= c) P[r * kPanelLd + c] -= P[r * kPanelLd + j] * P[c * kPanelLd + j];
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < R * 16; e += NT) {
+      const int r = e >> 4, c = e & 15;
+      if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = P[r * kPanelLd + c];
+    }
+    // trailing update of tiles (I, J), k < J <= I < nt
+    const int tn = nt - k - 1, T = tn * (tn + 1) / 2;
+    for (int idx = w; idx < T; idx += W) {
+      int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+      while ((a + 1) * (a + 2) / 2 <= idx) ++a;
+      while (a * (a + 1) / 2 > idx) --a;
+      const int I = k + 1 + a, J = k + 1 + (idx - a * (a + 1) / 2);
+      d4v acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
+        acc[r] = (i < m && jj < m) ? M[i * m + jj] : 0.0;
+      }
+      const double* pa = P + (16 * (I - k) + cq) * kPanelLd + rq;
+      const double* pb = P + (16 * (J - k) + cq) * kPanelLd + rq;
+#pragma unroll
+      for (int sl = 0; sl < 4; ++sl) {
+        const bool ia = 16 * (I - k) + cq < R, ib = 16 * (J - k) + cq < R;
+        const double av = ia ? -pa[4 * sl] : 0.0, bv = ib ? pb[4 * sl] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
+        if (i < m && jj < m) M[i * m + jj] = acc[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Blocked forward substitution L y = b (L from block_cholesky[_tiled]; b overwritten): wavefront 0
+// eliminates inside each 16-row block with lane broadcasts, then all threads apply the block to
+// the rows below it.
+template <int NT>
+__device__ void block_fsub_tiled(const double* L, const double* rcp, int m, double* b) {
+  const int tid = threadIdx.x, l = tid & 63;
+  const int nt = (m + 15) / 16;
+  for (int k = 0; k < nt; ++k) {
+    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
+    if (tid < 64) {
+      double bi = l < cw ? b[c0 + l] : 0.0;
+      for (int j = 0; j < cw; ++j) {
+        const double yj = __shfl(bi, j) * rcp[c0 + j];
+        const double lv = (l > j && l < cw) ? L[(c0 + l) * m + c0 + j] : 0.0;
+        bi = (l == j) ? yj : bi - lv * yj;
+      }
+      if (l < cw) b[c0 + l] = bi;
+    }
+    __syncthreads();
+    for (int r = c0 + 16 + tid; r < m; r += NT) {
+      const double* lr = L + r * m + c0;
+      double a0 = 0.0, a1 = 0.0;
+      int j = 0;
+      for (; j + 2 <= cw; j += 2) { a0 += lr[j] * b[c0 + j]; a1 += lr[j + 1] * b[c0 + j + 1]; }
+      if (j < cw) a0 += lr[j] * b[c0 + j];
+      b[r] -= a0 + a1;
+    }
+    __syncthreads();
+  }
+}
+
+// Blocked back substitution L^T x = b (b overwritten), blocks from the last.
+template <int NT>
+__device__ void block_bsub_t_tiled(const double* L, const double* rcp, int m, double* b) {
+  const int tid = threadIdx.x, l = tid & 63;
+  const int nt = (m + 15) / 16;
+  for (int k = nt - 1; k >= 0; --k) {
+    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
+    if (tid < 64) {
+      double bi = l < cw ? b[c0 + l] : 0.0;
+      for (int j = cw - 1; j >= 0; --j) {
+        const double xj = __shfl(bi, j) * rcp[c0 + j];
+        const double lv = (l < j) ? L[(c0 + j) * m + c0 + l] : 0.0;
+        bi = (l == j) ? xj : bi - lv * xj;
+      }
+      if (l < cw) b[c0 + l] = bi;
+    }
+    __syncthreads();
+    for (int r = tid; r < c0; r += NT) {
+      double a0 = 0.0, a1 = 0.0;
+      int j = 0;
+      for (; j + 2 <= cw; j += 2) {
+        a0 += L[(c0 + j) * m + r] * b[c0 + j];
+        a1 += L[(c0 + j + 1) * m + r] * b[c0 + j + 1];
+      }
+      if (j < cw) a0 += L[(c0 + j) * m + r] * b[c0 + j];
+      b[r] -= a0 + a1;
+    }
+    __syncthreads();
+  }
+}
+
+// W = L^{-1} A^T (n x me, row-major) by blocked forward substitution with me right-hand sides:
+// wavefront 0 eliminates each 16-row block (one lane per column of W), then every wavefront
+// applies the block to its share of the rows below.
+template <int NT>
+__device__ void block_w_tiled(const double* L, const double* rcp, int n, int me, double* Wm) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int nt = (n + 15) / 16;
+  for (int k = 0; k < nt; ++k) {
+    const int c0 = 16 * k, cw = n - c0 < 16 ? n - c0 : 16;
+    if (w == 0) {
+      for (int cg = 0; cg < me; cg += 64) {
+        const int c = cg + l;
+        double wj[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          double wi = (i < cw && c < me) ? Wm[(c0 + i) * me + c] : 0.0;
+#pragma unroll
+          for (int j = 0; j < i; ++j) wi -= (i < cw ? L[(c0 + i) * n + c0 + j] : 0.0) * wj[j];
+          wi *= i < cw ? rcp[c0 + i] : 0.0;
+          wj[i] = wi;
+          if (i < cw && c < me) Wm[(c0 + i) * me + c] = wi;
+        }
+      }
+    }
+    __syncthreads();
+    for (int cg = 0; cg < me; cg += 64) {
+      const int c = cg + l;
+      if (c < me) {
+        for (int r = c0 + 16 + w; r < n; r += W) {
+          const double* lr = L + r * n + c0;
+          double a0 = 0.0, a1 = 0.0;
+          int j = 0;
+          for (; j + 2 <= cw; j += 2) {
+            a0 += lr[j] * Wm[(c0 + j) * me + c];
+            a1 += lr[j + 1] * Wm[(c0 + j + 1) * me + c];
+          }
+          if (j < cw) a0 += lr[j] * Wm[(c0 + j) * me + c];
+          Wm[r * me + c] -= a0 + a1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Triangular solves with a factor from block_cholesky[_tiled]: blocked from m = 64 on.
+template <int NT>
+__device__ void block_fsub(const double* L, const double* rcp, int m, double* b) {
+  if (m >= kMfmaMinN) block_fsub_tiled<NT>(L, rcp, m, b);
+  else block_fsub_scalar<NT>(L, rcp, m, b);
+}
+
+template <int NT>
+__device__ void block_bsub_t(const double* L, const double* rcp, int m, double* b) {
+  if (m >= kMfmaMinN) block_bsub_t_tiled<NT>(L, rcp, m, b);
+  else block_bsub_t_scalar<NT>(L, rcp, m, b);
+}
+
 template <int NT>
 __device__ void stage_factor(const View& v, int dmode) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
@@ -284,7 +477,13 @@ __device__ void stage_factor(const View& v, int dmode) {
     }
   }
   __syncthreads();
-  block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
+  const bool tiled = n >= kMfmaMinN && n <= kMfmaMaxN;
+  if (tiled) {
+    extern __shared__ __align__(16) unsigned char qpx_smem[];
+    block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, reinterpret_cast<double*>(qpx_smem));
+  } else {
+    block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
+  }
   if (me == 0) return;
   // W = L^{-1} A^T, all me columns at once: column-oriented forward substitution.
   for (int e = tid; e < n * me; e += NT) {
@@ -292,7 +491,8 @@ __device__ void stage_factor(const View& v, int dmode) {
     v.W[e] = v.A[c * n + i];
   }
   __syncthreads();
-  for (int j = 0; j < n; ++j) {
+  if (tiled) block_w_tiled<NT>(v.H, v.hrcp, n, me, v.W);
+  for (int j = 0; j < (tiled ? 0 : n); ++j) {
     // row j of W is final once rows < j have been eliminated; scale it after its last use
     const double rj = v.hrcp[j];
     for (int e = tid; e < (n - j - 1) * me; e += NT) {
