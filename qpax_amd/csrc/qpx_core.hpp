@@ -198,7 +198,7 @@ enum DMode : int { kDzs = 0, kDunit = 1, kDgiven = 2 };
 constexpr int kMfmaMinN = 64;
 constexpr int kMfmaMaxN = 272;
 constexpr int kGtdgRows = 16;
-constexpr int kGtdgBatch = 8;
+constexpr int kGtdgBatch = 16;
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline int gtdg_stride(int n) { return 16 * (((n + 15) / 16) | 1); }
@@ -211,18 +211,79 @@ __host__ __device__ inline int gtdg_lds_bytes(int n) {
   return 8 * (a > b ? a : b);
 }
 
+// Sum over each group of 16 lanes (DPP row butterflies); every lane of the group receives it.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double sum16(double v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return v;
+}
+
+// out[k] = sum_j M[k n + j] x[j] for every row k < rows: 16 lanes per row read it coalesced and
+// meet in a DPP reduction (the large-size form of the row products, n >= kMfmaMinN).
+template <int NT>
+__device__ void rows_dot16(const double* M, int rows, int n, const double* x, double* out) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, rq = l >> 4, cq = l & 15;
+  for (int kb = 4 * w; kb < rows; kb += 4 * W) {
+    const int k = kb + rq;
+    const double* mr = M + (int64_t)(k < rows ? k : rows - 1) * n;
+    double a0 = 0.0, a1 = 0.0;
+    int j = cq;
+    for (; j + 16 < n; j += 32) {
+      a0 += mr[j] * x[j];
+      a1 += mr[j + 16] * x[j + 16];
+    }
+    if (j < n) a0 += mr[j] * x[j];
+    const double v = sum16(a0 + a1);
+    if (cq == 0 && k < rows) out[k] = v;
+  }
+}
+
 template <int NT>
 __device__ void gtdg_mfma(const View& v, double* chunk) {
   constexpr int W = NT / 64;
+  // 256-thread blocks prefetch the next chunk into registers under the MFMAs and keep 16
+  // accumulators per wavefront; smaller blocks (n <= 96) load in place and keep 8
+  constexpr bool kPf = NT >= 256;
+  constexpr int kBatch = kPf ? kGtdgBatch : kGtdgBatch / 2;
+  constexpr int EPT = kPf ? (kGtdgRows * (16 * ((kMfmaMaxN / 16) | 1)) + NT - 1) / NT : 1;
   const int n = v.n, mi = v.mi, tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
   const int nt = (n + 15) / 16, ldl = gtdg_stride(n), T = nt * (nt + 1) / 2;
+  const int csz = kGtdgRows * n;  // one chunk: kGtdgRows consecutive rows of G (row stride n)
   double* cD = chunk + kGtdgRows * ldl;
-  for (int base = 0; base < T; base += W * kGtdgBatch) {
-    int tI[kGtdgBatch], tJ[kGtdgBatch];
-    bool ok[kGtdgBatch];
+  // the chunk's elements this thread moves: flat index e = tid + NT u over kGtdgRows x n, placed
+  // at row e / n, column e % n of the LDS chunk (the columns n..ldl-1 are zeroed once per pass)
+  int dst[EPT];
 #pragma unroll
-    for (int b = 0; b < kGtdgBatch; ++b) {
+  for (int u = 0; u < EPT; ++u) {
+    const int e = tid + NT * u;
+    dst[u] = e < csz ? (e / n) * ldl + (e - (e / n) * n) : -1;
+  }
+  double reg[EPT], dreg = 0.0;
+  auto fetch = [&](int k0) {
+    if constexpr (!kPf) return;
+    const int64_t base = (int64_t)k0 * n, total = (int64_t)mi * n;
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int64_t g = base + tid + NT * u;
+      reg[u] = (dst[u] >= 0 && g < total) ? v.G[g] : 0.0;
+    }
+    dreg = (tid < kGtdgRows && k0 + tid < mi) ? v.D[k0 + tid] : 0.0;
+  };
+  for (int base = 0; base < T; base += W * kBatch) {
+    int tI[kBatch], tJ[kBatch];
+    bool ok[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
       const int idx = base + b * W + w;
       int I = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
       while ((I + 1) * (I + 2) / 2 <= idx) ++I;
@@ -231,33 +292,49 @@ __device__ void gtdg_mfma(const View& v, double* chunk) {
       tI[b] = ok[b] ? I : 0;
       tJ[b] = ok[b] ? idx - I * (I + 1) / 2 : 0;
     }
-    d4v acc[kGtdgBatch];
+    const int nb = (T - base - w + W - 1) / W;  // valid slots of this wavefront in the pass
+    d4v acc[kBatch];
 #pragma unroll
-    for (int b = 0; b < kGtdgBatch; ++b) acc[b] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < kBatch; ++b) acc[b] = d4v{0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    for (int e = tid; e < kGtdgRows * (ldl - n); e += NT) {
+      const int r = e / (ldl - n);
+      chunk[r * ldl + n + (e - r * (ldl - n))] = 0.0;
+    }
+    fetch(0);
     for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
       __syncthreads();
-      for (int r = 0; r < kGtdgRows; ++r) {
-        const int k = k0 + r;
-        for (int c = tid; c < ldl; c += NT) chunk[r * ldl + c] = (k < mi && c < n) ? v.G[(int64_t)k * n + c] : 0.0;
+      if constexpr (kPf) {
+#pragma unroll
+        for (int u = 0; u < EPT; ++u)
+          if (dst[u] >= 0) chunk[dst[u]] = reg[u];
+        if (tid < kGtdgRows) cD[tid] = dreg;
+      } else {
+        for (int r = 0; r < kGtdgRows; ++r) {
+          const int k = k0 + r;
+          for (int c = tid; c < n; c += NT) chunk[r * ldl + c] = k < mi ? v.G[(int64_t)k * n + c] : 0.0;
+        }
+        for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? v.D[k0 + r] : 0.0;
       }
-      for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? v.D[k0 + r] : 0.0;
       __syncthreads();
+      if (k0 + kGtdgRows < mi) fetch(k0 + kGtdgRows);  // in flight under the MFMAs
 #pragma unroll
       for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
         const double* gr = chunk + (4 * ks + rq) * ldl + cq;
         const double dk = cD[4 * ks + rq];
-        double a[kGtdgBatch], bb[kGtdgBatch];
+        double a[kBatch], bb[kBatch];
 #pragma unroll
-        for (int b = 0; b < kGtdgBatch; ++b) {
+        for (int b = 0; b < kBatch; ++b) {
           a[b] = gr[16 * tI[b]] * dk;
           bb[b] = gr[16 * tJ[b]];
         }
 #pragma unroll
-        for (int b = 0; b < kGtdgBatch; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
+        for (int b = 0; b < kBatch; ++b)
+          if (b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
       }
     }
 #pragma unroll
-    for (int b = 0; b < kGtdgBatch; ++b) {
+    for (int b = 0; b < kBatch; ++b) {
       if (!ok[b]) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -564,9 +641,16 @@ __device__ void stage_solve(const View& v, const double* a1, const double* a2, c
   for (int c = tid; c < me; c += NT) v.dy[c] = sg * a4[c];
   __syncthreads();
   reduced_solve<NT>(v, v.dy, dx);
+  const bool wide = n >= kMfmaMinN;
+  if (wide) {  // G dx into dz first
+    rows_dot16<NT>(v.G, mi, n, dx, dz);
+    __syncthreads();
+  }
   for (int k = tid; k < mi; k += NT) {
     double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * dx[j];
+    if (wide) acc = dz[k];
+    else
+      for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * dx[j];
     const double dsk = sg * a3[k] - acc;
     ds[k] = dsk;
     dz[k] = (sg * a2[k] - v.z[k] * dsk) / v.s[k];
@@ -581,10 +665,19 @@ __device__ void stage_solve(const View& v, const double* a1, const double* a2, c
 template <int NT>
 __device__ int stage_residuals(const View& v, double kappa, double tol, double* red) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  const bool wide = n >= kMfmaMinN;
+  if (wide) {  // G x into r3 and A x into r4 first; Q x by columns (Q is symmetric), coalesced
+    rows_dot16<NT>(v.G, mi, n, v.x, v.r3);
+    rows_dot16<NT>(v.A, me, n, v.x, v.r4);
+    __syncthreads();
+  }
   double m = 0.0;
   for (int i = tid; i < n; i += NT) {
     double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
+    if (wide)
+      for (int j = 0; j < n; ++j) acc += v.Q[j * n + i] * v.x[j];
+    else
+      for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
     acc += v.q[i];
     for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
     for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * v.z[k];
@@ -593,7 +686,9 @@ __device__ int stage_residuals(const View& v, double kappa, double tol, double* 
   }
   for (int k = tid; k < mi; k += NT) {
     double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * v.x[j];
+    if (wide) acc = v.r3[k];
+    else
+      for (int j = 0; j < n; ++j) acc += v.G[k * n + j] * v.x[j];
     const double r3 = acc + v.s[k] - v.h[k];
     const double r2 = v.s[k] * v.z[k] - kappa;
     v.r2[k] = r2;
@@ -602,7 +697,9 @@ __device__ int stage_residuals(const View& v, double kappa, double tol, double* 
   }
   for (int c = tid; c < me; c += NT) {
     double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc += v.A[c * n + j] * v.x[j];
+    if (wide) acc = v.r4[c];
+    else
+      for (int j = 0; j < n; ++j) acc += v.A[c * n + j] * v.x[j];
     const double r4 = acc - v.b[c];
     v.r4[c] = r4;
     m = nmax(m, fabs(r4));

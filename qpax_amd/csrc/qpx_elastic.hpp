@@ -111,10 +111,19 @@ template <int NT>
 __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2, double tol, double* red) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
+  const bool wide = n >= kMfmaMinN;
+  if (wide) {  // G x into r6 and A x into r4 first; Q x by columns (Q is symmetric), coalesced
+    rows_dot16<NT>(v.G, mi, n, v.x, e.r6);
+    rows_dot16<NT>(v.A, me, n, v.x, v.r4);
+    __syncthreads();
+  }
   double m = 0.0;
   for (int i = tid; i < n; i += NT) {
     double acc = v.q[i];
-    for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
+    if (wide)
+      for (int j = 0; j < n; ++j) acc += v.Q[j * n + i] * v.x[j];
+    else
+      for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
     for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * e.z2[k];
     for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
     v.r1[i] = acc;
@@ -122,7 +131,9 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
   }
   for (int k = tid; k < mi; k += NT) {
     double g = 0.0;
-    for (int j = 0; j < n; ++j) g += v.G[k * n + j] * v.x[j];
+    if (wide) g = e.r6[k];
+    else
+      for (int j = 0; j < n; ++j) g += v.G[k * n + j] * v.x[j];
     const double r2 = sign_r2 * (-e.z1[k] - e.z2[k] + e.pen);
     const double r3 = e.s1[k] * e.z1[k] - kappa;
     const double r4 = e.s2[k] * e.z2[k] - kappa;
@@ -133,7 +144,9 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
   }
   for (int c = tid; c < me; c += NT) {
     double acc = -v.b[c];
-    for (int j = 0; j < n; ++j) acc += v.A[c * n + j] * v.x[j];
+    if (wide) acc += v.r4[c];
+    else
+      for (int j = 0; j < n; ++j) acc += v.A[c * n + j] * v.x[j];
     v.r4[c] = acc;
     m = nmax(m, fabs(acc));
   }
@@ -174,9 +187,16 @@ __device__ void e_solve(EView& e, const double* R1, const double* R2, const doub
   for (int c = tid; c < me; c += NT) v.dy[c] = sg * R7[c];
   __syncthreads();
   reduced_solve<NT>(v, v.dy, v.dx);
+  const bool wide = n >= kMfmaMinN;
+  if (wide) {  // G dx into dz2 first
+    rows_dot16<NT>(v.G, mi, n, v.dx, dz2);
+    __syncthreads();
+  }
   for (int k = tid; k < mi; k += NT) {
     double g = 0.0;
-    for (int j = 0; j < n; ++j) g += v.G[k * n + j] * v.dx[j];
+    if (wide) g = dz2[k];
+    else
+      for (int j = 0; j < n; ++j) g += v.G[k * n + j] * v.dx[j];
     const double a3 = e.s1[k] / e.z1[k] + e.s2[k] / e.z2[k];
     const double d2 = (e.p1[k] + g) / a3;
     const double d1 = -(sg * R2[k]) - d2;
