@@ -135,12 +135,18 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
                                      const TS* q, const TS* A, const TS* b, const TS* G, const TS* h) {
   const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l;
   TS* Gs = const_cast<TS*>(c.Gs);
-  for (int k = 0; k < mi; ++k) {
-    const bool inf = isinf((double)h[k]);
-    for (int j = l; j < n; j += 64) {
-      const TS g = G[k * n + j];
-      Gs[k * n + j] = inf ? (TS)0 * g : g;  // diag(1 * ~mask) @ G (pdip.py:176)
+  // G into LDS as one flat copy, 8 loads in flight per lane
+  {
+    const int tot = mi * n;
+    int e = l;
+    for (; e + 7 * 64 < tot; e += 8 * 64) {
+      TS g[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) g[u] = G[e + 64 * u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Gs[e + 64 * u] = g[u];
     }
+    for (; e < tot; e += 64) Gs[e] = G[e];
   }
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -151,18 +157,40 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
     st.s[cc] = 1.0;
     st.z[cc] = 1.0;
   }
+  WSYNC();
+  // rows with infinite h: diag(1 * ~mask) @ G (pdip.py:176), 0 * g
+#pragma unroll
+  for (int cc = 0; cc < MIC; ++cc) {
+    uint64_t mask = __ballot(st.hinf[cc]);
+    while (mask) {
+      const int k = 64 * cc + __builtin_ctzll(mask);
+      mask &= mask - 1;
+      for (int j = l; j < n; j += 64) Gs[k * n + j] = (TS)0 * Gs[k * n + j];
+    }
+  }
+  // K's constant part, 8 columns per trip with their loads in flight
   if (l < m) {
-    for (int j = 0; j < m; ++j) {
-      double v = 0.0;
-      if (l < n) {
-        if (j < n) v = 0.5 * ((double)Q[l * n + j] + (double)Q[j * n + l]);
-        else if (j >= np) v = (double)A[(j - np) * n + l];
-      } else if (l < np) {
-        v = (j == l) ? 1.0 : 0.0;
-      } else if (j < n) {
-        v = (double)A[(l - np) * n + j];
+    for (int j0 = 0; j0 < m; j0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + u;
+        double val = 0.0;
+        if (j < m) {
+          if (l < n) {
+            if (j < n) val = 0.5 * ((double)Q[l * n + j] + (double)Q[j * n + l]);
+            else if (j >= np) val = (double)A[(j - np) * n + l];
+          } else if (l < np) {
+            val = (j == l) ? 1.0 : 0.0;
+          } else if (j < n) {
+            val = (double)A[(l - np) * n + j];
+          }
+        }
+        v[u] = val;
       }
-      c.KQ[j * m + l] = v;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (j0 + u < m) c.KQ[(j0 + u) * m + l] = v[u];
     }
   }
   __threadfence_block();
