@@ -175,15 +175,23 @@ __device__ __forceinline__ void load(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& 
                      const TS* b, const TS* G, const TS* h) {
   constexpr int NT = 64 * W, MIC = mic_of(W);
   const int n = c.n, np = c.np, mi = c.mi, m = c.m, t = c.t, ldg = c.ldg;
-  for (int k = 0; k < mi; ++k) {
-    const bool inf = isinf((double)h[k]);
+  // the padded workspace copy of G, 8 rows per trip with their loads in flight
+  for (int k0 = 0; k0 < mi; k0 += 8) {
     for (int j = t; j < ldg; j += NT) {
-      TS v = (TS)0;
-      if (j < n) {
-        const TS g = G[(int64_t)k * n + j];
-        v = inf ? (TS)0 * g : g;  // diag(1 * ~mask) @ G (pdip.py:176)
+      TS v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        v[u] = (k < mi && j < n) ? G[(int64_t)k * n + j] : (TS)0;
       }
-      c.Gw[(int64_t)k * ldg + j] = v;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        if (k < mi) {
+          const bool inf = isinf((double)h[k]);
+          c.Gw[(int64_t)k * ldg + j] = inf ? (TS)0 * v[u] : v[u];  // diag(1 * ~mask) @ G (pdip.py:176)
+        }
+      }
     }
   }
 #pragma unroll
@@ -195,18 +203,28 @@ __device__ __forceinline__ void load(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& 
     st.s[cc] = 1.0;
     st.z[cc] = 1.0;
   }
-  if (t < m) {
-    for (int j = 0; j < m; ++j) {
-      double v = 0.0;
-      if (t < n) {
-        if (j < n) v = 0.5 * ((double)Q[t * n + j] + (double)Q[j * n + t]);
-        else if (j >= np) v = (double)A[(j - np) * n + t];
-      } else if (t < np) {
-        v = (j == t) ? 1.0 : 0.0;
-      } else if (j < n) {
-        v = (double)A[(t - np) * n + j];
+  if (t < m) {  // K's constant part, 8 columns per trip
+    for (int j0 = 0; j0 < m; j0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + u;
+        double val = 0.0;
+        if (j < m) {
+          if (t < n) {
+            if (j < n) val = 0.5 * ((double)Q[t * n + j] + (double)Q[j * n + t]);
+            else if (j >= np) val = (double)A[(j - np) * n + t];
+          } else if (t < np) {
+            val = (j == t) ? 1.0 : 0.0;
+          } else if (j < n) {
+            val = (double)A[(t - np) * n + j];
+          }
+        }
+        v[u] = val;
       }
-      c.KQ[(int64_t)j * m + t] = v;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (j0 + u < m) c.KQ[(int64_t)(j0 + u) * m + t] = v[u];
     }
   }
   st.qa = t < n ? (double)q[t] : ((t >= np && t < m) ? -(double)b[t - np] : 0.0);
