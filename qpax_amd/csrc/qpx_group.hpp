@@ -44,7 +44,7 @@ __host__ __device__ inline int ntiles(int m) { return (m + 15) / 16; }
 __host__ __device__ inline int g_stride(int m) { return 16 * (ntiles(m) | 1); }
 
 struct GLds {
-  int off_L, off_A, off_B, off_pv, off_vb, off_vc, off_d0, off_red, off_part, bytes;
+  int off_L, off_A, off_B, off_pv, off_vb, off_vc, off_vz, off_d0, off_red, off_part, bytes;
 };
 
 // LDS carve-up for one problem (float64 words): the packed factor, which also holds the two G
@@ -67,6 +67,7 @@ __host__ __device__ inline GLds group_lds(int m, int mi, int NT, int ts) {
   w.off_pv = o; o += 16;
   w.off_vb = o; o += (mi + 1) & ~1;
   w.off_vc = o; o += (mi + 1) & ~1;
+  w.off_vz = o; o += (mi + KC - 1) / KC * KC;
   w.off_d0 = o; o += rows;
   w.off_red = o; o += 16;
   w.off_part = o; o += NT;
@@ -109,6 +110,7 @@ struct GCtx {
   double* pv;
   double* vb;
   double* vc;
+  double* vz;   // z (zero-padded to whole G chunks) for the fused residual products
   double* d0;
   double* red;
   double* part;
@@ -161,7 +163,7 @@ __device__ __forceinline__ GCtx<TS, NTI, W> make_gctx(int n, int me, int mi, dou
   c.Lp = base + L.off_L; c.Gst = reinterpret_cast<TS*>(c.Lp);
   c.pbA = base + L.off_A; c.col = c.pbA;
   c.pbB = base + L.off_B; c.pv = base + L.off_pv;
-  c.vb = base + L.off_vb; c.vc = base + L.off_vc;
+  c.vb = base + L.off_vb; c.vc = base + L.off_vc; c.vz = base + L.off_vz;
   c.d0 = base + L.off_d0; c.red = base + L.off_red; c.part = base + L.off_part;
   c.KQ = ws;
   c.Gw = reinterpret_cast<TS*>(ws + round_up((int64_t)c.m * c.m, 64));
@@ -297,24 +299,32 @@ __device__ __forceinline__ double g_col(const GCtx<TS, NTI, W>& c, const double*
   return col_dot<W>(c.Gw, c.ldg, c.mi, c.n, v, c.part);
 }
 
-// Residuals of the (relaxed) KKT system (qpax/pdip.py:93-100); returns the infinity norm.
+// Residuals of the (relaxed) KKT system (qpax/pdip.py:93-100) in two halves around the
+// factorisation, which streams G through LDS anyway and forms G x and G^T z on the way.
+// Before: x to the broadcast vector, z to LDS, and r1's constant part q + K_const [x; y].
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ double residuals(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, double kappa) {
+__device__ __forceinline__ void residual_pre(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st) {
   constexpr int NT = 64 * W, MIC = mic_of(W);
   const int m = c.m, mi = c.mi, t = c.t;
   if (t < c.ldg) c.col[t] = t < m ? st.xa : 0.0;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = NT * cc + t;
-    if (k < mi) c.vb[k] = st.z[cc];
+    if (k < (mi + KC - 1) / KC * KC) c.vz[k] = k < mi ? st.z[cc] : 0.0;
   }
   __syncthreads();
-  // K's constant part times [x; y] (KQ[j m + i] = K[i][j]), then G^T z
-  double acc = st.qa + col_dot<W>(c.KQ, m, m, m, c.col, c.part);
-  acc += g_col(c, c.vb);
-  st.ra = t < m ? acc : 0.0;
-  g_rows(c, c.vc);
-  __syncthreads();
+  // K's constant part times [x; y] (KQ[j m + i] = K[i][j])
+  st.ra = st.qa + col_dot<W>(c.KQ, m, m, m, c.col, c.part);
+}
+
+// After: r1 = q + K_const [x; y] + G^T z (gz: this thread's column of G^T z), r2 = s z - kappa,
+// r3 = G x + s - h; returns the infinity norm (NaN-propagating).
+template <typename TS, int NTI, int W>
+__device__ __forceinline__ double residual_post(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, double gz,
+                                                double kappa) {
+  constexpr int NT = 64 * W, MIC = mic_of(W);
+  const int m = c.m, mi = c.mi, t = c.t;
+  st.ra = t < m ? st.ra + gz : 0.0;
   double nrm = fabs(st.ra);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -341,7 +351,8 @@ __device__ __forceinline__ double residuals(const GCtx<TS, NTI, W>& c, GSt<mic_o
 // the column values, which differs from the eliminated row values only in the last bit (and in
 // already finished columns, which are never read again).
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, bool unit_d) {
+__device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, bool unit_d,
+                                       double* gz = nullptr) {
   constexpr int NT = 64 * W, MIC = mic_of(W), SMAX = smax_of(NTI, W);
   const int np = c.np, mi = c.mi, m = c.m, t = c.t, ldg = c.ldg, rows = 16 * c.nti, nti = c.nti;
   const int rq = c.rq, cq = c.cq;
@@ -399,9 +410,28 @@ __device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>
     fetch(0);
     stash(0);
     __syncthreads();
+    // fused residual products (gz != nullptr): rows of G x, 16 lanes per row, into vc; the
+    // columns of G^T z split over NT / ceil64(n) thread groups, combined after the stream
+    const int cpn = (c.n + 63) & ~63, P = NT / cpn > 0 ? NT / cpn : 1, jc = t % cpn, pt = t / cpn;
+    double gzacc = 0.0;
     for (int ch = 0; ch < nch; ++ch) {
       if (ch + 1 < nch) fetch(ch + 1);
       const TS* Gc = c.Gst + (ch & 1) * csz;
+      if (gz) {
+        for (int kb = 4 * c.w; kb < KC; kb += 4 * W) {
+          const int kk = kb + rq, k = ch * KC + kk;
+          const TS* gr = Gc + kk * ldg + cq;
+          double a0 = 0.0;
+#pragma unroll
+          for (int u = 0; u < NTI; ++u)
+            if (u < c.nti) a0 += (double)gr[16 * u] * c.col[16 * u + cq];
+          a0 = row16_sum(a0);
+          if (cq == 0 && k < mi) c.vc[k] = a0;
+        }
+        if (pt < P && jc < c.n) {
+          for (int kk = pt; kk < KC; kk += P) gzacc += (double)Gc[kk * ldg + jc] * c.vz[ch * KC + kk];
+        }
+      }
       double a[SMAX], b[SMAX];
       auto ld = [&](int ks, double (&aa)[SMAX], double (&bb)[SMAX]) {
         const int kk = 4 * ks + rq;
@@ -427,6 +457,15 @@ __device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>
         }
       }
       if (ch + 1 < nch) stash((ch + 1) & 1);
+      __syncthreads();
+    }
+    if (gz) {
+      if (pt < P) c.part[pt * cpn + jc] = gzacc;
+      __syncthreads();
+      double v = 0.0;
+      if (t < c.n)
+        for (int q = 0; q < P; ++q) v += c.part[q * cpn + t];
+      *gz = v;
       __syncthreads();
     }
   }
@@ -757,8 +796,10 @@ template <typename TS, int NTI, int W>
 __device__ __forceinline__ int pc_iter(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, double tol) {
   constexpr int NT = 64 * W, MIC = mic_of(W);
   const int mi = c.mi, t = c.t;
-  const double nrm = residuals(c, st, 0.0);
-  factor(c, st, false);
+  residual_pre(c, st);
+  double gz;
+  factor(c, st, false, &gz);
+  const double nrm = residual_post(c, st, gz, 0.0);
   solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.dsa, st.dza);
   // centering (qpax/pdip.py:76-84)
   double sz = 0.0;
@@ -785,8 +826,10 @@ __device__ __forceinline__ int pc_iter(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>
 // One relaxed Newton iteration (qpax/pdip_relaxed.py:7-52).
 template <typename TS, int NTI, int W>
 __device__ __forceinline__ int relax_iter(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, double tol, double kappa) {
-  const double nrm = residuals(c, st, kappa);
-  factor(c, st, false);
+  residual_pre(c, st);
+  double gz;
+  factor(c, st, false, &gz);
+  const double nrm = residual_post(c, st, gz, kappa);
   solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
   step(c, st, true);
   return nrm < tol ? 1 : 0;
