@@ -56,6 +56,19 @@ __device__ __forceinline__ double wsum(double v) { return wreduce(v, OpSum()); }
 
 #define WSYNC() __builtin_amdgcn_wave_barrier()
 
+// Copies of the lane index and of the problem sizes that the compiler cannot hoist out of the
+// iteration loops: lane masks and LDS addresses derived from them are recomputed per call instead
+// of being kept live in registers across the whole solve.
+__device__ __forceinline__ int opaque_v(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ int opaque_s(int x) {
+  x = __builtin_amdgcn_readfirstlane(x);
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
 // sum_k G[k][col] * v[k] over k < mi (v broadcast from LDS); four rows per trip.
 template <typename TS>
 __device__ __forceinline__ double gcol_dot(const TS* __restrict__ Gs, int n, int mi, int col,
@@ -203,7 +216,7 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
 // (NaN-propagating).
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double kappa) {
-  const int n = c.n, mi = c.mi, m = c.m, l = c.l;
+  const int n = opaque_s(c.n), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
   c.col[l] = st.xa;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -263,7 +276,7 @@ template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool unit_d) {
   constexpr int NTI = NMAX / 16;
   constexpr int NTILE = NTI * (NTI + 1) / 2;
-  const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l;
+  const int n = opaque_s(c.n), np = opaque_s(c.np), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
   const int rq = l >> 4, cq = l & 15;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -326,91 +339,134 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
   }
   WSYNC();
   double* pb = c.pb;
-  // Diagonal of the not yet eliminated part: lanes holding diagonal entries of the diagonal tiles
-  // publish them, every lane reads its row's.
-  auto snap_diag = [&](int from) {
+  // Reference diagonal of this lane's row: the entry before elimination.  Schur rows (l >= np)
+  // measure their pivots against the entry once all H pivots are done, i.e. minus the sum of
+  // lf^2 * pivot over the H pivots, accumulated in hsum as the pivots go by.
 #pragma unroll
-    for (int I = 0; I < NTI; ++I) {
+  for (int I = 0; I < NTI; ++I) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (4 * r + rq == cq) pb[16 * I + cq] = T[tidx(I, I)][r];
-    }
-    WSYNC();
-    const double v = pb[l];
-    WSYNC();
-    return v;
-  };
-  double d0 = snap_diag(0);
-  // Panels of 4 pivots.
+    for (int r = 0; r < 4; ++r)
+      if (4 * r + rq == cq) pb[16 * I + cq] = T[tidx(I, I)][r];
+  }
+  WSYNC();
+  const double dref = pb[l];
+  WSYNC();
+  double hsum = 0.0;
+  // Tile columns of 16 pivots, each as 4 panels of 4.  A panel's rank-4 update reaches the
+  // column's own tiles at once; the tiles right of the column receive its 16 pivots as one
+  // rank-16 update (four MFMAs per tile) when the column is done, with the multipliers read back
+  // from the packed factor and the pivots from its diagonal slots (which the sweeps never read).
+  // Pivots c >= m meet zero rows and columns: they fall below the floor and only ever scale
+  // zeros.
 #pragma unroll
-  for (int p = 0; p < NMAX / 4; ++p) {
-    if (4 * p >= m) continue;
-    constexpr int dummy = 0;
-    (void)dummy;
-    const int J = p >> 2, jc = 4 * (p & 3), c0 = 4 * p;
-    if (c0 == np) {  // first pivot of the Schur block: its reference diagonal
-      const double v = snap_diag(0);
-      if (l >= np) d0 = v;
-    }
-    // extract the panel's 4 columns, one row per lane
+  for (int J = 0; J < NTI; ++J) {
+    if (16 * J >= m) break;
 #pragma unroll
-    for (int I = 0; I < NTI; ++I) {
-      if (I < J) continue;
+    for (int pp = 0; pp < 4; ++pp) {
+      const int c0 = 16 * J + 4 * pp, jc = 4 * pp;
+      // floor of each pivot of the panel: kPivotRel times its row's reference diagonal
+      const double rf = kPivotRel * fabs(l >= np ? dref - hsum : dref);
+      double thr[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (cq >= jc && cq < jc + 4) pb[(16 * I + 4 * r + rq) * 4 + (cq - jc)] = T[tidx(I, J)][r];
-    }
-    WSYNC();
-    double v[4];
-    {
-      const double2 v01 = *reinterpret_cast<const double2*>(pb + l * 4);
-      const double2 v23 = *reinterpret_cast<const double2*>(pb + l * 4 + 2);
-      v[0] = v01.x; v[1] = v01.y; v[2] = v23.x; v[3] = v23.y;
-    }
-    WSYNC();
-    // the 4 pivots in sequence
-    double lf[4], cv[4];
+      for (int t = 0; t < 4; ++t) thr[t] = rl(rf, c0 + t);
+      // extract the panel's 4 columns, one row per lane
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int cc = c0 + t;
-      if (cc < m) {
-        const double dj = rl(v[t], cc);
-        const double d0j = rl(d0, cc);
-        const bool hp = cc < np;
-        const bool ok = hp ? (dj > kPivotRel * fabs(d0j)) : (dj < -kPivotRel * fabs(d0j));
-        const double inv = recip(ok ? dj : (hp ? kHugePivot : -kHugePivot));
+      for (int I = 0; I < NTI; ++I) {
+        if (I < J) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (cq >= jc && cq < jc + 4) pb[(16 * I + 4 * r + rq) * 4 + (cq - jc)] = T[tidx(I, J)][r];
+      }
+      WSYNC();
+      // this lane's row and the 4 x 4 diagonal block, which every lane eliminates for itself
+      double v[4], V[4][4];
+      {
+        const double2 v01 = *reinterpret_cast<const double2*>(pb + l * 4);
+        const double2 v23 = *reinterpret_cast<const double2*>(pb + l * 4 + 2);
+        v[0] = v01.x; v[1] = v01.y; v[2] = v23.x; v[3] = v23.y;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double2 a01 = *reinterpret_cast<const double2*>(pb + (c0 + i) * 4);
+          const double2 a23 = *reinterpret_cast<const double2*>(pb + (c0 + i) * 4 + 2);
+          V[i][0] = a01.x; V[i][1] = a01.y; V[i][2] = a23.x; V[i][3] = a23.y;
+        }
+      }
+      WSYNC();
+      double lf[4], pe[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int cc = c0 + t;
+        const double sgn = cc < np ? 1.0 : -1.0;
+        const double dj = V[t][t];
+        const bool ok = sgn * dj > thr[t];
+        pe[t] = ok ? dj : sgn * kHugePivot;
+        const double inv = recip(pe[t]);
         if (l == cc) st.rdpiv = inv;
-        cv[t] = v[t];
         lf[t] = (l > cc) ? v[t] * inv : 0.0;
 #pragma unroll
-        for (int u = t + 1; u < 4; ++u) v[u] -= lf[t] * rl(v[t], c0 + u);
-      } else {
-        cv[t] = 0.0;
-        lf[t] = 0.0;
+        for (int u = t + 1; u < 4; ++u) v[u] -= lf[t] * V[u][t];
+#pragma unroll
+        for (int i = t + 1; i < 4; ++i) {
+          const double li = V[i][t] * inv;
+#pragma unroll
+          for (int u = t + 1; u <= i; ++u) V[i][u] -= li * V[u][t];
+        }
+        if (cc < np) hsum += lf[t] * lf[t] * pe[t];
       }
+      // multipliers into the packed factor, pivots into its diagonal slots
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (l > c0 + t && l < m) c.Lp[l * (l + 1) / 2 + c0 + t] = lf[t];
+      if (l < 4 && c0 + l < m) {
+        const double pl = l == 0 ? pe[0] : l == 1 ? pe[1] : l == 2 ? pe[2] : pe[3];
+        c.Lp[(c0 + l) * (c0 + l + 3) / 2] = pl;
+      }
+      *reinterpret_cast<double2*>(pb + l * 8) = double2{lf[0], lf[1]};
+      *reinterpret_cast<double2*>(pb + l * 8 + 2) = double2{lf[2], lf[3]};
+      *reinterpret_cast<double2*>(pb + l * 8 + 4) = double2{lf[0] * pe[0], lf[1] * pe[1]};
+      *reinterpret_cast<double2*>(pb + l * 8 + 6) = double2{lf[2] * pe[2], lf[3] * pe[3]};
+      WSYNC();
+      // rank-4 update of the column's own tiles (the last panel of a column has none left)
+      if (pp < 3) {
+        const double bJ = pb[(16 * J + cq) * 8 + 4 + rq];
+        double aA[NTI];
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) aA[I] = (I >= J) ? -pb[(16 * I + cq) * 8 + rq] : 0.0;
+#pragma unroll
+        for (int I = 0; I < NTI; ++I)
+          if (I >= J) T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(aA[I], bJ, T[tidx(I, J)], 0, 0, 0);
+      }
+      WSYNC();
     }
-    // multipliers into the packed factor; rank-4 trailing update through the matrix cores
+    // the column's 16 pivots on the tiles right of it, the next column's tiles first
+    if (J + 1 < NTI) {
+      double Lx[NTI][4], pv[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-      if (l > c0 + t && l < m) c.Lp[l * (l + 1) / 2 + c0 + t] = lf[t];
-    *reinterpret_cast<double2*>(pb + l * 8) = double2{lf[0], lf[1]};
-    *reinterpret_cast<double2*>(pb + l * 8 + 2) = double2{lf[2], lf[3]};
-    *reinterpret_cast<double2*>(pb + l * 8 + 4) = double2{cv[0], cv[1]};
-    *reinterpret_cast<double2*>(pb + l * 8 + 6) = double2{cv[2], cv[3]};
-    WSYNC();
-    double aA[NTI], bB[NTI];
+      for (int kq = 0; kq < 4; ++kq) {
+        const int cc = 16 * J + 4 * kq + rq, ccl = cc < m ? cc : m - 1;
+        pv[kq] = cc < m ? c.Lp[ccl * (ccl + 3) / 2] : 0.0;
+      }
 #pragma unroll
-    for (int I = 0; I < NTI; ++I) {
-      aA[I] = (I >= J) ? -pb[(16 * I + cq) * 8 + rq] : 0.0;
-      bB[I] = (I >= J) ? pb[(16 * I + cq) * 8 + 4 + rq] : 0.0;
+      for (int X = 0; X < NTI; ++X) {
+        const int row = 16 * X + cq, rr = row < m ? row : m - 1;
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq)
+          Lx[X][kq] = (X > J && row < m) ? c.Lp[rr * (rr + 1) / 2 + 16 * J + 4 * kq + rq] : 0.0;
+      }
+#pragma unroll
+      for (int Jp = 0; Jp < NTI; ++Jp) {
+        if (Jp <= J) continue;
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) {
+          if (I < Jp) continue;
+#pragma unroll
+          for (int kq = 0; kq < 4; ++kq)
+            T[tidx(I, Jp)] = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lx[I][kq], Lx[Jp][kq] * pv[kq], T[tidx(I, Jp)], 0,
+                                                                  0, 0);
+        }
+      }
+      WSYNC();
     }
-#pragma unroll
-    for (int I = 0; I < NTI; ++I) {
-#pragma unroll
-      for (int Jp = 0; Jp <= I; ++Jp)
-        if (Jp >= J) T[tidx(I, Jp)] = __builtin_amdgcn_mfma_f64_16x16x4f64(aA[I], bB[Jp], T[tidx(I, Jp)], 0, 0, 0);
-    }
-    WSYNC();
   }
 }
 
@@ -421,7 +477,7 @@ template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double pa,
                                       const double (&a2)[MIC], const double (&a3)[MIC], double sg,
                                       double (&ods)[MIC], double (&odz)[MIC]) {
-  const int n = c.n, mi = c.mi, m = c.m, l = c.l;
+  const int n = opaque_s(c.n), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
