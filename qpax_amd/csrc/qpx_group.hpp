@@ -170,6 +170,26 @@ __device__ __forceinline__ GCtx<TS, NTI, W> make_gctx(int n, int me, int mi, dou
   return c;
 }
 
+// Copy of the context whose sizes, thread indices and tile-slot tables the compiler cannot hoist
+// out of the iteration loops (each per-iteration stage takes one): lane masks and LDS addresses
+// derived from them are recomputed per stage instead of being kept live across the whole solve.
+template <typename TS, int NTI, int W>
+__device__ __forceinline__ GCtx<TS, NTI, W> fresh(const GCtx<TS, NTI, W>& c) {
+  constexpr int SMAX = smax_of(NTI, W);
+  GCtx<TS, NTI, W> o = c;
+  o.n = wave::opaque_s(c.n); o.np = wave::opaque_s(c.np); o.me = wave::opaque_s(c.me);
+  o.mi = wave::opaque_s(c.mi); o.m = wave::opaque_s(c.m); o.nti = wave::opaque_s(c.nti);
+  o.ldg = wave::opaque_s(c.ldg); o.w = wave::opaque_s(c.w);
+  o.t = wave::opaque_v(c.t); o.l = wave::opaque_v(c.l); o.rq = wave::opaque_v(c.rq); o.cq = wave::opaque_v(c.cq);
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    o.sI[s] = wave::opaque_s(c.sI[s]);
+    o.sJ[s] = wave::opaque_s(c.sJ[s]);
+    o.sv[s] = wave::opaque_s(c.sv[s] ? 1 : 0) != 0;
+  }
+  return o;
+}
+
 // Load and sanitise (qpax/pdip.py:166-178, :223): the workspace copy of G, K's constant part
 // [[Q_sym, A^T], [A, 0]] with identity padding rows, h, q and -b.
 template <typename TS, int NTI, int W>
@@ -238,7 +258,8 @@ __device__ __forceinline__ void load(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& 
 // out[k] = sum_j G[k][j] * col[j] for every row k < mi (col zero beyond m); 16 lanes per row,
 // two rows per lane group per trip so that twice the loads are in flight.
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ void g_rows(const GCtx<TS, NTI, W>& c, double* out) {
+__device__ __forceinline__ void g_rows(const GCtx<TS, NTI, W>& cin, double* out) {
+  const GCtx<TS, NTI, W> c = fresh(cin);
   const int mi = c.mi, ldg = c.ldg, nti = c.nti, rq = c.rq, cq = c.cq;
   for (int kb = 4 * c.w; kb < mi; kb += 8 * W) {
     const int k0 = kb + rq, k1 = kb + 4 * W + rq;
@@ -295,7 +316,8 @@ __device__ __forceinline__ double col_dot(const TM* M, int64_t ld, int R, int C,
 }
 
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ double g_col(const GCtx<TS, NTI, W>& c, const double* v) {
+__device__ __forceinline__ double g_col(const GCtx<TS, NTI, W>& cin, const double* v) {
+  const GCtx<TS, NTI, W> c = fresh(cin);
   return col_dot<W>(c.Gw, c.ldg, c.mi, c.n, v, c.part);
 }
 
@@ -303,7 +325,8 @@ __device__ __forceinline__ double g_col(const GCtx<TS, NTI, W>& c, const double*
 // factorisation, which streams G through LDS anyway and forms G x and G^T z on the way.
 // Before: x to the broadcast vector, z to LDS, and r1's constant part q + K_const [x; y].
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ void residual_pre(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st) {
+__device__ __forceinline__ void residual_pre(const GCtx<TS, NTI, W>& cin, GSt<mic_of(W)>& st) {
+  const GCtx<TS, NTI, W> c = fresh(cin);
   constexpr int NT = 64 * W, MIC = mic_of(W);
   const int m = c.m, mi = c.mi, t = c.t;
   if (t < c.ldg) c.col[t] = t < m ? st.xa : 0.0;
@@ -320,8 +343,9 @@ __device__ __forceinline__ void residual_pre(const GCtx<TS, NTI, W>& c, GSt<mic_
 // After: r1 = q + K_const [x; y] + G^T z (gz: this thread's column of G^T z), r2 = s z - kappa,
 // r3 = G x + s - h; returns the infinity norm (NaN-propagating).
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ double residual_post(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, double gz,
+__device__ __forceinline__ double residual_post(const GCtx<TS, NTI, W>& cin, GSt<mic_of(W)>& st, double gz,
                                                 double kappa) {
+  const GCtx<TS, NTI, W> c = fresh(cin);
   constexpr int NT = 64 * W, MIC = mic_of(W);
   const int m = c.m, mi = c.mi, t = c.t;
   st.ra = t < m ? st.ra + gz : 0.0;
@@ -351,8 +375,9 @@ __device__ __forceinline__ double residual_post(const GCtx<TS, NTI, W>& c, GSt<m
 // the column values, which differs from the eliminated row values only in the last bit (and in
 // already finished columns, which are never read again).
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, bool unit_d,
+__device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& cin, GSt<mic_of(W)>& st, bool unit_d,
                                        double* gz = nullptr) {
+  const GCtx<TS, NTI, W> c = fresh(cin);
   constexpr int NT = 64 * W, MIC = mic_of(W), SMAX = smax_of(NTI, W);
   const int np = c.np, mi = c.mi, m = c.m, t = c.t, ldg = c.ldg, rows = 16 * c.nti, nti = c.nti;
   const int rq = c.rq, cq = c.cq;
@@ -599,9 +624,10 @@ __device__ __forceinline__ void factor(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>
 // Solve K [dx; dy] = [p1; v4] (qpax/pdip.py:49-63) with v1 = sg*a1 (augmented with v4 = sg*a4 in
 // rows np..m-1: the argument pa), v2 = sg*a2, v3 = sg*a3.  Writes st.da and the ds/dz chunks.
 template <typename TS, int NTI, int W>
-__device__ __forceinline__ void solve(const GCtx<TS, NTI, W>& c, GSt<mic_of(W)>& st, double pa,
+__device__ __forceinline__ void solve(const GCtx<TS, NTI, W>& cin, GSt<mic_of(W)>& st, double pa,
                                       const double (&a2)[mic_of(W)], const double (&a3)[mic_of(W)], double sg,
                                       double (&ods)[mic_of(W)], double (&odz)[mic_of(W)]) {
+  const GCtx<TS, NTI, W> c = fresh(cin);
   constexpr int NT = 64 * W, MIC = mic_of(W);
   const int mi = c.mi, m = c.m, t = c.t, w = c.w, l = c.l;
 #pragma unroll
