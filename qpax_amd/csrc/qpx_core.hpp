@@ -23,6 +23,19 @@ constexpr double kHugePivot = 1e64;    // a pivot below the floor is replaced by
 __device__ __forceinline__ double nmin(double a, double b) { return (a < b || a != a) ? a : b; }
 __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
 
+// Copies of a lane index or of a uniform size that the compiler cannot hoist out of the
+// iteration loops: lane masks and addresses derived from them are recomputed per stage instead of
+// being kept live in registers across the whole solve.
+__device__ __forceinline__ int opaque_v(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ int opaque_s(int x) {
+  x = __builtin_amdgcn_readfirstlane(x);
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
 struct OpMin { __device__ double operator()(double a, double b) const { return nmin(a, b); } };
 struct OpMax { __device__ double operator()(double a, double b) const { return nmax(a, b); } };
 struct OpSum { __device__ double operator()(double a, double b) const { return a + b; } };

@@ -177,15 +177,15 @@ template <typename TS, int NTI, int W>
 __device__ __forceinline__ GCtx<TS, NTI, W> fresh(const GCtx<TS, NTI, W>& c) {
   constexpr int SMAX = smax_of(NTI, W);
   GCtx<TS, NTI, W> o = c;
-  o.n = wave::opaque_s(c.n); o.np = wave::opaque_s(c.np); o.me = wave::opaque_s(c.me);
-  o.mi = wave::opaque_s(c.mi); o.m = wave::opaque_s(c.m); o.nti = wave::opaque_s(c.nti);
-  o.ldg = wave::opaque_s(c.ldg); o.w = wave::opaque_s(c.w);
-  o.t = wave::opaque_v(c.t); o.l = wave::opaque_v(c.l); o.rq = wave::opaque_v(c.rq); o.cq = wave::opaque_v(c.cq);
+  o.n = opaque_s(c.n); o.np = opaque_s(c.np); o.me = opaque_s(c.me);
+  o.mi = opaque_s(c.mi); o.m = opaque_s(c.m); o.nti = opaque_s(c.nti);
+  o.ldg = opaque_s(c.ldg); o.w = opaque_s(c.w);
+  o.t = opaque_v(c.t); o.l = opaque_v(c.l); o.rq = opaque_v(c.rq); o.cq = opaque_v(c.cq);
 #pragma unroll
   for (int s = 0; s < SMAX; ++s) {
-    o.sI[s] = wave::opaque_s(c.sI[s]);
-    o.sJ[s] = wave::opaque_s(c.sJ[s]);
-    o.sv[s] = wave::opaque_s(c.sv[s] ? 1 : 0) != 0;
+    o.sI[s] = opaque_s(c.sI[s]);
+    o.sJ[s] = opaque_s(c.sJ[s]);
+    o.sv[s] = opaque_s(c.sv[s] ? 1 : 0) != 0;
   }
   return o;
 }

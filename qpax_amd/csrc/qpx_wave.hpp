@@ -53,21 +53,16 @@ __device__ __forceinline__ double wreduce(double v, Op op) {
 __device__ __forceinline__ double wmax(double v) { return wreduce(v, OpMax()); }
 __device__ __forceinline__ double wmin(double v) { return wreduce(v, OpMin()); }
 __device__ __forceinline__ double wsum(double v) { return wreduce(v, OpSum()); }
+// Sum over the 16 lanes of each DPP row; every lane of the row receives it.
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp<0xB1>(v);
+  v += dpp<0x4E>(v);
+  v += dpp<0x141>(v);
+  v += dpp<0x140>(v);
+  return v;
+}
 
 #define WSYNC() __builtin_amdgcn_wave_barrier()
-
-// Copies of the lane index and of the problem sizes that the compiler cannot hoist out of the
-// iteration loops: lane masks and LDS addresses derived from them are recomputed per call instead
-// of being kept live in registers across the whole solve.
-__device__ __forceinline__ int opaque_v(int x) {
-  asm volatile("" : "+v"(x));
-  return x;
-}
-__device__ __forceinline__ int opaque_s(int x) {
-  x = __builtin_amdgcn_readfirstlane(x);
-  asm volatile("" : "+s"(x));
-  return x;
-}
 
 // sum_k G[k][col] * v[k] over k < mi (v broadcast from LDS); four rows per trip.
 template <typename TS>
@@ -272,19 +267,32 @@ __host__ __device__ constexpr int tidx(int I, int J) { return I * (I + 1) / 2 + 
 // beyond kPivotRel times its reference diagonal -- the diagonal entry before elimination for the
 // H block, the diagonal entry once all H pivots are done for the Schur block -- or is NaN is
 // replaced by kHugePivot with the block's sign, which zeroes that component of later solves.
-template <typename TS, int NMAX, int MIC>
-__device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool unit_d) {
+//
+// With RES the same pass forms the residuals of the (relaxed) KKT system at the current point
+// (qpax/pdip.py:93-100): K's constant part times [x; y] from the tiles before G^T D G enters
+// them, and G x and G^T z from the G operands the G^T D G stream loads anyway (vector ALU work
+// under the matrix-core work).  Returns their infinity norm (NaN-propagating), 0 without RES.
+template <bool RES, typename TS, int NMAX, int MIC>
+__device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool unit_d,
+                                         double kappa) {
   constexpr int NTI = NMAX / 16;
   constexpr int NTILE = NTI * (NTI + 1) / 2;
   const int n = opaque_s(c.n), np = opaque_s(c.np), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
   const int rq = l >> 4, cq = l & 15;
+  double* vz = c.vb + mi;  // z, for G^T z
+  double* kx = c.col + 64; // K's constant part times [x; y], then plus G^T z
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     st.D[cc] = unit_d ? 1.0 : st.z[cc] / st.s[cc];
-    if (k < mi) c.vb[k] = st.D[cc];
+    if (k < mi) {
+      c.vb[k] = st.D[cc];
+      if (RES) vz[k] = st.z[cc];
+    }
   }
-  // tiles start as K's constant part
+  if (RES) c.col[l] = st.xa;
+  // tiles start as K's constant part (symmetric; with RES read along its rows, coalesced over the
+  // lanes)
   d4 T[NTILE];
 #pragma unroll
   for (int I = 0; I < NTI; ++I) {
@@ -294,15 +302,53 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * I + 4 * r + rq, cl = 16 * J + cq;
         const bool in = row < m && cl < m;
-        const double v = c.KQ[(in ? cl : 0) * m + (in ? row : 0)];
+        const double v = RES ? c.KQ[(in ? row : 0) * m + (in ? cl : 0)] : c.KQ[(in ? cl : 0) * m + (in ? row : 0)];
         T[tidx(I, J)][r] = in ? v : 0.0;
       }
     }
   }
   WSYNC();
+  // [x; y] at this lane's tile columns (xc) and tile rows (xr)
+  double xc[NTI], pc[NTI], zacc[NTI];
+#pragma unroll
+  for (int J = 0; J < NTI; ++J) {
+    xc[J] = RES ? c.col[16 * J + cq] : 0.0;
+    pc[J] = 0.0;
+    zacc[J] = 0.0;
+  }
+  if (RES) {
+    // K's constant part times [x; y]: the stored lower tiles act directly on their rows and, off
+    // the diagonal, transposed on their columns.  Row sums over the 16 lanes of a DPP row, column
+    // sums over the four lane groups of 16.
+    double pr[NTI][4];
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double xr = c.col[16 * I + 4 * r + rq];
+        double a = 0.0;
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+          a += T[tidx(I, J)][r] * xc[J];
+          if (J < I) pc[J] += T[tidx(I, J)][r] * xr;
+        }
+        pr[I][r] = a;
+      }
+    }
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double v = row16_sum(pr[I][r]);
+        if (cq == 0) kx[16 * I + 4 * r + rq] = v;
+      }
+    }
+  }
   // G^T D G into the tiles, four rows of G per step, the next step's operands loaded while the
   // current step's MFMAs run.  Columns >= n load as exact zeros.  No branch may separate the
-  // MFMAs (the accumulators would be copied around each one).
+  // MFMAs (the accumulators would be copied around each one).  With RES: the rows of G x (sum
+  // over the 16 lanes of a DPP row, stored over the D entries already consumed) and this lane's
+  // share of G^T z.
   {
     int off[NTI];
     double msk[NTI];
@@ -312,19 +358,20 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
       off[I] = cl < n ? cl : n - 1;
       msk[I] = cl < n ? 1.0 : 0.0;
     }
-    auto fetch = [&](int k0, double (&g)[NTI], double& dk) {
+    auto fetch = [&](int k0, double (&g)[NTI], double& dk, double& zk) {
       const int k = k0 + rq;
       const int kc = k < mi ? k : mi - 1;
       dk = k < mi ? c.vb[kc] : 0.0;
+      zk = (RES && k < mi) ? vz[kc] : 0.0;
       const TS* grow = c.Gs + kc * n;
 #pragma unroll
       for (int I = 0; I < NTI; ++I) g[I] = (double)grow[off[I]] * msk[I];
     };
-    double gc[NTI], dc;
-    fetch(0, gc, dc);
+    double gc[NTI], dc, zc;
+    fetch(0, gc, dc, zc);
     for (int k0 = 0; k0 < mi; k0 += 4) {
-      double gn[NTI], dn;
-      fetch(k0 + 4 < mi ? k0 + 4 : k0, gn, dn);
+      double gn[NTI], dn, zn;
+      fetch(k0 + 4 < mi ? k0 + 4 : k0, gn, dn, zn);
 #pragma unroll
       for (int I = 0; I < NTI; ++I) {
         const double a = gc[I] * dc;
@@ -332,12 +379,56 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
         for (int J = 0; J <= I; ++J)
           T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, gc[J], T[tidx(I, J)], 0, 0, 0);
       }
+      if (RES) {
+        double gx = 0.0;
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) {
+          gx += gc[I] * xc[I];
+          zacc[I] += gc[I] * zc;
+        }
+        gx = row16_sum(gx);
+        const int k = k0 + rq;
+        if (cq == 0 && k < mi) c.vb[k] = gx;
+      }
 #pragma unroll
       for (int I = 0; I < NTI; ++I) gc[I] = gn[I];
       dc = dn;
+      zc = zn;
     }
   }
   WSYNC();
+  double nrm = 0.0;
+  if (RES) {
+    // columns of the off-diagonal tiles and of G^T z: sum over the lane groups, add to kx
+#pragma unroll
+    for (int J = 0; J < NTI; ++J) {
+      double v = pc[J] + zacc[J];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      pc[J] = v;
+    }
+    if (rq == 0) {
+#pragma unroll
+      for (int J = 0; J < NTI; ++J) kx[16 * J + cq] += pc[J];
+    }
+    WSYNC();
+    st.ra = l < m ? kx[l] + st.qa : 0.0;
+    nrm = fabs(st.ra);
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      if (k < mi) {
+        st.r3[cc] = c.vb[k] + st.s[cc] - st.h[cc];
+        st.r2[cc] = st.s[cc] * st.z[cc] - kappa;
+        nrm = nmax(nrm, nmax(fabs(st.r2[cc]), fabs(st.r3[cc])));
+      } else {
+        st.r3[cc] = 0.0;
+        st.r2[cc] = 0.0;
+      }
+    }
+    WSYNC();
+    nrm = wmax(nrm);
+  }
   double* pb = c.pb;
   // Reference diagonal of this lane's row: the entry before elimination.  Schur rows (l >= np)
   // measure their pivots against the entry once all H pivots are done, i.e. minus the sum of
@@ -468,6 +559,7 @@ __device__ __forceinline__ void factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
       WSYNC();
     }
   }
+  return nrm;
 }
 
 // Solve K [dx; dy] = [p1; v4] (qpax/pdip.py:49-63) with v1 = sg*a1 (augmented with v4 = sg*a4 in
@@ -580,7 +672,7 @@ __device__ __forceinline__ double ort(const double (&x)[MIC], const double (&dx)
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void init(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st) {
   const int mi = c.mi, l = c.l;
-  factor(c, st, true);
+  factor<false>(c, st, true, 0.0);
   // p = [-q + G^T h; b]: with a2 = 0, a3 = h, sg = 1 the solve forms p1 = v1 + G^T h
   double zero[MIC];
 #pragma unroll
@@ -622,11 +714,16 @@ __device__ __forceinline__ void step(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
 }
 
 // One predictor-corrector iteration (qpax/pdip.py:87-119); returns the entry convergence flag.
-template <typename TS, int NMAX, int MIC>
+template <bool FRES, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double tol) {
   const int mi = c.mi, l = c.l;
-  const double nrm = residuals(c, st, 0.0);
-  factor(c, st, false);
+  double nrm;
+  if constexpr (FRES) {
+    nrm = factor<true>(c, st, false, 0.0);
+  } else {
+    nrm = residuals(c, st, 0.0);
+    factor<false>(c, st, false, 0.0);
+  }
   solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.dsa, st.dza);
   // centering (qpax/pdip.py:76-84)
   double sz = 0.0;
@@ -650,11 +747,16 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
 }
 
 // One relaxed Newton iteration (qpax/pdip_relaxed.py:7-52).
-template <typename TS, int NMAX, int MIC>
+template <bool FRES, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double tol,
                                           double kappa) {
-  const double nrm = residuals(c, st, kappa);
-  factor(c, st, false);
+  double nrm;
+  if constexpr (FRES) {
+    nrm = factor<true>(c, st, false, kappa);
+  } else {
+    nrm = residuals(c, st, kappa);
+    factor<false>(c, st, false, 0.0);
+  }
   solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
   step(c, st, true);
   return nrm < tol ? 1 : 0;
@@ -714,8 +816,10 @@ __device__ __forceinline__ int64_t next_problem(const Args<TS>& a, int& round) {
   return a.order ? (int64_t)a.order[idx] : idx;
 }
 
-// Forward kernel: solve (and optionally relax) QPs, one per wavefront at a time.
-template <typename TS, int NMAX, int MIC>
+// Forward kernel: solve (and optionally relax) QPs, one per wavefront at a time.  FRES selects
+// the residuals formed inside the factorisation's G^T D G pass (uniform batches) over the separate
+// residual pass (ragged batches).
+template <typename TS, int NMAX, int MIC, bool FRES>
 __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) {
   int round = 0;
   for (int64_t p; (p = next_problem(a, round)) >= 0;) {
@@ -727,7 +831,7 @@ __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) 
     init(c, st);
     int conv = 0, it = 0;
     while (it < kMaxIters && conv == 0) {
-      conv = pc_iter(c, st, a.tol);
+      conv = pc_iter<FRES>(c, st, a.tol);
       ++it;
     }
     store_state(c, st, pi, a.x, a.s, a.z, a.y, true);
@@ -739,7 +843,7 @@ __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) 
       if (st.hinf[cc]) { st.z[cc] = 0.0; st.s[cc] = INFINITY; }
     int rconv = 0, rit = 0;
     while (rit < kMaxIters && rconv == 0) {
-      rconv = relax_iter(c, st, a.tol, a.kappa);
+      rconv = relax_iter<FRES>(c, st, a.tol, a.kappa);
       ++rit;
     }
     store_state(c, st, pi, a.xr, a.sr, a.zr, a.yr, false);
@@ -766,7 +870,7 @@ __global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
       st.s[cc] = k < mi ? (double)a.sr_in[pi.oh + k] : 1.0;
       st.z[cc] = k < mi ? (double)a.zr_in[pi.oh + k] : 1.0;
     }
-    factor(c, st, false);
+    factor<false>(c, st, false, 0.0);
     double zero[MIC];
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;

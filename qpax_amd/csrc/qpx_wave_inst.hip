@@ -9,7 +9,12 @@ namespace wave {
 // `grid` workgroups of `wpb` wavefronts; a.lds_per_wave bytes of LDS per wavefront.
 template <typename TS, int NMAX, int MIC>
 hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st, int relax) {
-  hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a, relax);
+  if (a.rg)
+    hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC, false>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a,
+                       relax);
+  else
+    hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC, true>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a,
+                       relax);
   return hipGetLastError();
 }
 
@@ -19,11 +24,12 @@ hipError_t launch_wave_bwd(const Args<TS>& a, unsigned grid, int wpb, hipStream_
   return hipGetLastError();
 }
 
-// Resident workgroups of `wpb` wavefronts per CU (forward kernel, the longer-running of the two).
+// Resident workgroups of `wpb` wavefronts per CU (forward kernel as ragged batches run it, the
+// longer-running of the two).
 template <typename TS, int NMAX, int MIC>
 int wave_occupancy(int wpb, int lds_bytes) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_wave_solve<TS, NMAX, MIC>),
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_wave_solve<TS, NMAX, MIC, false>),
                                                    64 * wpb, (size_t)lds_bytes * wpb) != hipSuccess)
     nb = 0;
   return nb;
