@@ -562,6 +562,246 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
   return nrm;
 }
 
+// The same factorisation and residuals on the upper triangle of K, the form the uniform batches
+// use.  Tile (J, I), J <= I, holds element (16J + 4r + l/16, 16I + l%16) in register r of lane l,
+// so the 4 pivot rows of a panel are one register of each tile of the panel's tile row, and that
+// register is already the B operand (and, scaled by 1/pivot per lane group, the A operand) of
+// v_mfma_f64_16x16x4 for the panel's rank-4 update of every trailing tile: no data movement
+// between the elimination and the matrix cores.  Inside a panel the pivot rows are eliminated
+// from each other with lane broadcasts.  The trailing updates act on whole tiles; entries of
+// already eliminated rows and columns only ever receive rounding-level values and are never read.
+template <bool RES, typename TS, int NMAX, int MIC>
+__device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool unit_d,
+                                            double kappa) {
+  constexpr int NTI = NMAX / 16;
+  constexpr int NTILE = NTI * (NTI + 1) / 2;
+  const int n = opaque_s(c.n), np = opaque_s(c.np), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
+  const int rq = l >> 4, cq = l & 15;
+  double* vz = c.vb + mi;  // z, for G^T z
+  double* kx = c.col + 64; // K's constant part times [x; y], then plus G^T z
+#pragma unroll
+  for (int cc = 0; cc < MIC; ++cc) {
+    const int k = 64 * cc + l;
+    st.D[cc] = unit_d ? 1.0 : st.z[cc] / st.s[cc];
+    if (k < mi) {
+      c.vb[k] = st.D[cc];
+      if (RES) vz[k] = st.z[cc];
+    }
+  }
+  if (RES) c.col[l] = st.xa;
+  // tiles start as K's constant part, read along its rows (coalesced over the lanes)
+  d4 T[NTILE];
+#pragma unroll
+  for (int I = 0; I < NTI; ++I) {
+#pragma unroll
+    for (int J = 0; J <= I; ++J) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * J + 4 * r + rq, cl = 16 * I + cq;
+        const bool in = row < m && cl < m;
+        const double v = c.KQ[(in ? row : 0) * m + (in ? cl : 0)];
+        T[tidx(I, J)][r] = in ? v : 0.0;
+      }
+    }
+  }
+  WSYNC();
+  double xc[NTI], pc[NTI], zacc[NTI];
+#pragma unroll
+  for (int I = 0; I < NTI; ++I) {
+    xc[I] = RES ? c.col[16 * I + cq] : 0.0;
+    pc[I] = 0.0;
+    zacc[I] = 0.0;
+  }
+  if (RES) {
+    // K's constant part times [x; y]: the stored upper tiles act directly on their rows and, off
+    // the diagonal, transposed on their columns
+    double pr[NTI][4];
+#pragma unroll
+    for (int J = 0; J < NTI; ++J) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double xr = c.col[16 * J + 4 * r + rq];
+        double a = 0.0;
+#pragma unroll
+        for (int I = J; I < NTI; ++I) {
+          a += T[tidx(I, J)][r] * xc[I];
+          if (I > J) pc[I] += T[tidx(I, J)][r] * xr;
+        }
+        pr[J][r] = a;
+      }
+    }
+#pragma unroll
+    for (int J = 0; J < NTI; ++J) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double v = row16_sum(pr[J][r]);
+        if (cq == 0) kx[16 * J + 4 * r + rq] = v;
+      }
+    }
+  }
+  // G^T D G into the tiles (operands as in factor, A from the tile row, B from the tile column),
+  // with G x and G^T z under the MFMAs when RES
+  {
+    int off[NTI];
+    double msk[NTI];
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+      const int cl = 16 * I + cq;
+      off[I] = cl < n ? cl : n - 1;
+      msk[I] = cl < n ? 1.0 : 0.0;
+    }
+    auto fetch = [&](int k0, double (&g)[NTI], double& dk, double& zk) {
+      const int k = k0 + rq;
+      const int kc = k < mi ? k : mi - 1;
+      dk = k < mi ? c.vb[kc] : 0.0;
+      zk = (RES && k < mi) ? vz[kc] : 0.0;
+      const TS* grow = c.Gs + kc * n;
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) g[I] = (double)grow[off[I]] * msk[I];
+    };
+    double gc[NTI], dc, zc;
+    fetch(0, gc, dc, zc);
+    for (int k0 = 0; k0 < mi; k0 += 4) {
+      double gn[NTI], dn, zn;
+      fetch(k0 + 4 < mi ? k0 + 4 : k0, gn, dn, zn);
+#pragma unroll
+      for (int J = 0; J < NTI; ++J) {
+        const double a = gc[J] * dc;
+#pragma unroll
+        for (int I = J; I < NTI; ++I)
+          T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, gc[I], T[tidx(I, J)], 0, 0, 0);
+      }
+      if (RES) {
+        double gx = 0.0;
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) {
+          gx += gc[I] * xc[I];
+          zacc[I] += gc[I] * zc;
+        }
+        gx = row16_sum(gx);
+        const int k = k0 + rq;
+        if (cq == 0 && k < mi) c.vb[k] = gx;
+      }
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) gc[I] = gn[I];
+      dc = dn;
+      zc = zn;
+    }
+  }
+  WSYNC();
+  double nrm = 0.0;
+  if (RES) {
+#pragma unroll
+    for (int I = 0; I < NTI; ++I) {
+      double v = pc[I] + zacc[I];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      pc[I] = v;
+    }
+    if (rq == 0) {
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) kx[16 * I + cq] += pc[I];
+    }
+    WSYNC();
+    st.ra = l < m ? kx[l] + st.qa : 0.0;
+    nrm = fabs(st.ra);
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      if (k < mi) {
+        st.r3[cc] = c.vb[k] + st.s[cc] - st.h[cc];
+        st.r2[cc] = st.s[cc] * st.z[cc] - kappa;
+        nrm = nmax(nrm, nmax(fabs(st.r2[cc]), fabs(st.r3[cc])));
+      } else {
+        st.r3[cc] = 0.0;
+        st.r2[cc] = 0.0;
+      }
+    }
+    WSYNC();
+    nrm = wmax(nrm);
+  }
+  double* pb = c.pb;
+  // reference diagonal of this lane's row: the entry before elimination; for Schur rows it is
+  // replaced by the entry once all H pivots are done (the tiles are current at that panel)
+  auto snap_diag = [&]() {
+#pragma unroll
+    for (int J = 0; J < NTI; ++J) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * r + rq == cq) pb[16 * J + cq] = T[tidx(J, J)][r];
+    }
+    WSYNC();
+    const double v = pb[l];
+    WSYNC();
+    return v;
+  };
+  double dref = snap_diag();
+#pragma unroll
+  for (int J = 0; J < NTI; ++J) {
+    if (16 * J >= m) break;
+#pragma unroll
+    for (int pp = 0; pp < 4; ++pp) {
+      const int c0 = 16 * J + 4 * pp, jc = 4 * pp;
+      if (c0 == np) {
+        const double v = snap_diag();
+        if (l >= np) dref = v;
+      }
+      const double rf = kPivotRel * fabs(dref);
+      double thr[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) thr[t] = rl(rf, c0 + t);
+      // the panel's pivot rows: register pp of the tiles of tile row J (row c0 + rq in lane group rq)
+      double P[NTI];
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) P[I] = I >= J ? T[tidx(I, J)][pp] : 0.0;
+      double inv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int cc = c0 + t;
+        const double sgn = cc < np ? 1.0 : -1.0;
+        const double dj = rl(P[J], 16 * t + jc + t);  // K'[cc][cc]
+        const double pe = (sgn * dj > thr[t]) ? dj : sgn * kHugePivot;
+        inv[t] = recip(pe);
+        if (l == cc) st.rdpiv = inv[t];
+        if (t < 3) {
+          // rows c0 + t' (t' > t) lose their column-cc multiple of row c0 + t
+          double mv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) mv[u] = u > t ? rl(P[J], 16 * u + jc + t) * inv[t] : 0.0;
+          const double mult = rq == 1 ? mv[1] : rq == 2 ? mv[2] : rq == 3 ? mv[3] : 0.0;
+#pragma unroll
+          for (int I = 0; I < NTI; ++I) {
+            if (I < J) continue;
+            const double rowt = __shfl(P[I], 16 * t + cq, 64);
+            P[I] -= mult * rowt;
+          }
+        }
+      }
+      // this lane group's pivot reciprocal; multipliers L[16I + cq][c0 + rq] into the packed factor
+      const double myinv = rq == 0 ? inv[0] : rq == 1 ? inv[1] : rq == 2 ? inv[2] : inv[3];
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) {
+        if (I < J) continue;
+        const int row = 16 * I + cq;
+        if (row > c0 + rq && row < m) c.Lp[row * (row + 1) / 2 + c0 + rq] = P[I] * myinv;
+      }
+      // rank-4 update of every trailing tile: A = -L rows (P of the tile row, scaled), B = U rows
+      // (P of the tile column); the panel's own tile row first, which the next panel reads.  The
+      // last panel of a tile row leaves that row's tiles alone.
+#pragma unroll
+      for (int Jp = 0; Jp < NTI; ++Jp) {
+        if (Jp < J || (Jp == J && pp == 3)) continue;
+        const double a = -P[Jp] * myinv;
+#pragma unroll
+        for (int I = 0; I < NTI; ++I)
+          if (I >= Jp) T[tidx(I, Jp)] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, P[I], T[tidx(I, Jp)], 0, 0, 0);
+      }
+    }
+  }
+  WSYNC();
+  return nrm;
+}
+
 // Solve K [dx; dy] = [p1; v4] (qpax/pdip.py:49-63) with v1 = sg*a1 (augmented with v4 = sg*a4 in
 // rows np..m-1: the argument pa), v2 = sg*a2, v3 = sg*a3.  Writes st.da (dx; 0; dy) and the given
 // ds/dz chunks.
@@ -669,10 +909,11 @@ __device__ __forceinline__ double ort(const double (&x)[MIC], const double (&dx)
   return wmin(a);
 }
 
-template <typename TS, int NMAX, int MIC>
+template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void init(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st) {
   const int mi = c.mi, l = c.l;
-  factor<false>(c, st, true, 0.0);
+  if constexpr (FAST) factor_up<false>(c, st, true, 0.0);
+  else factor<false>(c, st, true, 0.0);
   // p = [-q + G^T h; b]: with a2 = 0, a3 = h, sg = 1 the solve forms p1 = v1 + G^T h
   double zero[MIC];
 #pragma unroll
@@ -719,7 +960,7 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
   const int mi = c.mi, l = c.l;
   double nrm;
   if constexpr (FRES) {
-    nrm = factor<true>(c, st, false, 0.0);
+    nrm = factor_up<true>(c, st, false, 0.0);
   } else {
     nrm = residuals(c, st, 0.0);
     factor<false>(c, st, false, 0.0);
@@ -752,7 +993,7 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
                                           double kappa) {
   double nrm;
   if constexpr (FRES) {
-    nrm = factor<true>(c, st, false, kappa);
+    nrm = factor_up<true>(c, st, false, kappa);
   } else {
     nrm = residuals(c, st, kappa);
     factor<false>(c, st, false, 0.0);
@@ -817,8 +1058,8 @@ __device__ __forceinline__ int64_t next_problem(const Args<TS>& a, int& round) {
 }
 
 // Forward kernel: solve (and optionally relax) QPs, one per wavefront at a time.  FRES selects
-// the residuals formed inside the factorisation's G^T D G pass (uniform batches) over the separate
-// residual pass (ragged batches).
+// factor_up with the residuals formed inside its G^T D G pass (uniform batches) over the separate
+// residual pass and factor (ragged batches).
 template <typename TS, int NMAX, int MIC, bool FRES>
 __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) {
   int round = 0;
@@ -828,7 +1069,7 @@ __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) 
     const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + pi.ows, a.lds_per_wave);
     St<TS, NMAX, MIC> st;
     load(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
-    init(c, st);
+    init<FRES>(c, st);
     int conv = 0, it = 0;
     while (it < kMaxIters && conv == 0) {
       conv = pc_iter<FRES>(c, st, a.tol);
@@ -852,8 +1093,9 @@ __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) 
   }
 }
 
-// Backward kernel (qpax/diff_qp.py:20-46 with optnet_derivatives :8-17).
-template <typename TS, int NMAX, int MIC>
+// Backward kernel (qpax/diff_qp.py:20-46 with optnet_derivatives :8-17); FAST selects factor_up
+// (uniform batches) over factor (ragged batches).
+template <typename TS, int NMAX, int MIC, bool FAST>
 __global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
   int round = 0;
   for (int64_t p; (p = next_problem(a, round)) >= 0;) {
@@ -870,7 +1112,8 @@ __global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
       st.s[cc] = k < mi ? (double)a.sr_in[pi.oh + k] : 1.0;
       st.z[cc] = k < mi ? (double)a.zr_in[pi.oh + k] : 1.0;
     }
-    factor<false>(c, st, false, 0.0);
+    if constexpr (FAST) factor_up<false>(c, st, false, 0.0);
+    else factor<false>(c, st, false, 0.0);
     double zero[MIC];
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;

@@ -20,7 +20,10 @@ hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStrea
 
 template <typename TS, int NMAX, int MIC>
 hipError_t launch_wave_bwd(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st) {
-  hipLaunchKernelGGL((k_wave_bwd<TS, NMAX, MIC>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a);
+  if (a.rg)
+    hipLaunchKernelGGL((k_wave_bwd<TS, NMAX, MIC, false>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a);
+  else
+    hipLaunchKernelGGL((k_wave_bwd<TS, NMAX, MIC, true>), dim3(grid), dim3(64 * wpb), a.lds_per_wave * wpb, st, a);
   return hipGetLastError();
 }
 

@@ -72,4 +72,10 @@ def test_exact_dG_matches_oracle():
     xo, rel, info = O.solve_qp_primal_fwd(*data, solver_tol=1e-6, target_kappa=1e-3)
     go = O.solve_qp_primal_bwd(*data, rel, 2 * (xo - 1), exact_dG=True)
     ok = (info[0] == 1) & (info[2] == 1)
-    assert rel_err(to_np(ts[4].grad)[ok], go[4][ok]).max() < 1e-7
+    # at solver_tol 1e-6 the reduced KKT matrix is conditioned so that a different elimination's
+    # rounding moves the relaxed point (and dG with it) by up to ~1e-6 on a few problems
+    e = rel_err(to_np(ts[4].grad)[ok], go[4][ok])
+    assert e.max() < 1e-6 and np.median(e) < 1e-8
+    # and the exact form is told apart from the reference's combination (off by percents)
+    gref = O.solve_qp_primal_bwd(*data, rel, 2 * (xo - 1), exact_dG=False)[4]
+    assert np.median(rel_err(to_np(ts[4].grad)[ok], gref[ok])) > 1e-3
