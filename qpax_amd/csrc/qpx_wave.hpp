@@ -567,8 +567,9 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
 // so the 4 pivot rows of a panel are one register of each tile of the panel's tile row, and that
 // register is already the B operand (and, scaled by 1/pivot per lane group, the A operand) of
 // v_mfma_f64_16x16x4 for the panel's rank-4 update of every trailing tile: no data movement
-// between the elimination and the matrix cores.  Inside a panel the pivot rows are eliminated
-// from each other with lane broadcasts.  The trailing updates act on whole tiles; entries of
+// between the elimination and the matrix cores.  Inside a panel every lane eliminates the 4 x 4
+// diagonal block (read with lane broadcasts) for itself, and the pivot rows follow as
+// U = L_block^{-1} R from lane-group broadcasts of the original rows, off the pivot chain.  The trailing updates act on whole tiles; entries of
 // already eliminated rows and columns only ever receive rounding-level values and are never read.
 template <bool RES, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool unit_d,
@@ -754,29 +755,46 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       double P[NTI];
 #pragma unroll
       for (int I = 0; I < NTI; ++I) P[I] = I >= J ? T[tidx(I, J)][pp] : 0.0;
-      double inv[4];
+      // the panel's 4 x 4 diagonal block as uniform values, and the broadcasts of its first three
+      // rows to every lane group (both depend on the panel's input only)
+      double V[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) V[i][u] = rl(P[J], 16 * i + jc + u);
+      double R[3][NTI];
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) R[t][I] = I >= J ? __shfl(P[I], 16 * t + cq, 64) : 0.0;
+      // eliminate the block: pivots, and multipliers lm[u][t] of the unit-lower block factor
+      double inv[4], lm[4][4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int cc = c0 + t;
         const double sgn = cc < np ? 1.0 : -1.0;
-        const double dj = rl(P[J], 16 * t + jc + t);  // K'[cc][cc]
+        const double dj = V[t][t];
         const double pe = (sgn * dj > thr[t]) ? dj : sgn * kHugePivot;
         inv[t] = recip(pe);
         if (l == cc) st.rdpiv = inv[t];
-        if (t < 3) {
-          // rows c0 + t' (t' > t) lose their column-cc multiple of row c0 + t
-          double mv[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) mv[u] = u > t ? rl(P[J], 16 * u + jc + t) * inv[t] : 0.0;
-          const double mult = rq == 1 ? mv[1] : rq == 2 ? mv[2] : rq == 3 ? mv[3] : 0.0;
+        for (int u = t + 1; u < 4; ++u) {
+          lm[u][t] = V[u][t] * inv[t];
 #pragma unroll
-          for (int I = 0; I < NTI; ++I) {
-            if (I < J) continue;
-            const double rowt = __shfl(P[I], 16 * t + cq, 64);
-            P[I] -= mult * rowt;
-          }
+          for (int w = t + 1; w < 4; ++w) V[u][w] -= lm[u][t] * V[t][w];
         }
       }
+      // pivot rows U = L_block^{-1} R: row u gains sum_{t<u} Linv[u][t] * (original row t)
+      const double i10 = -lm[1][0], i21 = -lm[2][1], i32 = -lm[3][2];
+      const double i20 = -lm[2][0] - lm[2][1] * i10;
+      const double i31 = -lm[3][1] - lm[3][2] * i21;
+      const double i30 = -lm[3][0] - lm[3][1] * i10 - lm[3][2] * i20;
+      const double k0 = rq == 1 ? i10 : rq == 2 ? i20 : rq == 3 ? i30 : 0.0;
+      const double k1 = rq == 2 ? i21 : rq == 3 ? i31 : 0.0;
+      const double k2 = rq == 3 ? i32 : 0.0;
+#pragma unroll
+      for (int I = 0; I < NTI; ++I)
+        if (I >= J) P[I] += k0 * R[0][I] + k1 * R[1][I] + k2 * R[2][I];
       // this lane group's pivot reciprocal; multipliers L[16I + cq][c0 + rq] into the packed factor
       const double myinv = rq == 0 ? inv[0] : rq == 1 ? inv[1] : rq == 2 ? inv[2] : inv[3];
 #pragma unroll
