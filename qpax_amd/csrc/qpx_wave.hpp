@@ -823,7 +823,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
 // Solve K [dx; dy] = [p1; v4] (qpax/pdip.py:49-63) with v1 = sg*a1 (augmented with v4 = sg*a4 in
 // rows np..m-1: the argument pa), v2 = sg*a2, v3 = sg*a3.  Writes st.da (dx; 0; dy) and the given
 // ds/dz chunks.
-template <typename TS, int NMAX, int MIC>
+template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double pa,
                                       const double (&a2)[MIC], const double (&a3)[MIC], double sg,
                                       double (&ods)[MIC], double (&odz)[MIC]) {
@@ -867,23 +867,25 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
     }
   }
   pa = l < m ? pa * st.rdpiv : 0.0;
-  // backward: L^T x = y, column j of L^T is row j of L (rows clamped to m - 1 for the address)
+  // backward: L^T x = y, column j of L^T is row j of L (rows clamped to m - 1 for the address;
+  // FAST selects between the row's compile-time offset and the last row's, one scalar select per
+  // row instead of the clamp's arithmetic)
+  const int tri_last = (m - 1) * m / 2;
+  auto col_of = [&](int j) {
+    if constexpr (FAST) return c.Lp[(j < m ? j * (j + 1) / 2 : tri_last) + l];
+    const int jj = j < m ? j : m - 1;
+    return c.Lp[jj * (jj + 1) / 2 + l];
+  };
   {
     double lv[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int j = NMAX - 1 - u, jj = j < m ? j : m - 1;
-      lv[u] = c.Lp[jj * (jj + 1) / 2 + l];
-    }
+    for (int u = 0; u < 16; ++u) lv[u] = col_of(NMAX - 1 - u);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       double nv[16];
       if (b + 1 < NB) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = NMAX - 1 - (16 * (b + 1) + u), jj = j < m ? j : m - 1;
-          nv[u] = c.Lp[jj * (jj + 1) / 2 + l];
-        }
+        for (int u = 0; u < 16; ++u) nv[u] = col_of(NMAX - 1 - (16 * (b + 1) + u));
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
@@ -937,7 +939,7 @@ __device__ __forceinline__ void init(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;
   const double pa = -st.qa;  // -q in rows < n, b in rows np..m-1 (qa holds -b), 0 elsewhere
-  solve(c, st, pa, zero, st.h, 1.0, st.ds, st.dz);
+  solve<FAST>(c, st, pa, zero, st.h, 1.0, st.ds, st.dz);
   st.xa = st.da;
   // z = G x - h  ==  -(h - G x) = -ds from the solve above
   double zmax = -INFINITY, zmin = INFINITY;
@@ -983,7 +985,7 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
     nrm = residuals(c, st, 0.0);
     factor<false>(c, st, false, 0.0);
   }
-  solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.dsa, st.dza);
+  solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.dsa, st.dza);
   // centering (qpax/pdip.py:76-84)
   double sz = 0.0;
 #pragma unroll
@@ -1000,7 +1002,7 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
   const double sm = ratio * ratio * ratio * mu;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) st.r2[cc] = st.r2[cc] - (sm - st.dsa[cc] * st.dza[cc]);
-  solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
+  solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
   step(c, st, false);
   return nrm < tol ? 1 : 0;
 }
@@ -1016,7 +1018,7 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
     nrm = residuals(c, st, kappa);
     factor<false>(c, st, false, 0.0);
   }
-  solve(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
+  solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
   step(c, st, true);
   return nrm < tol ? 1 : 0;
 }
@@ -1136,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;
     const double g = l < n ? (double)a.g[pi.oq + l] : 0.0;
-    solve(c, st, -g, zero, zero, 1.0, st.ds, st.dz);
+    solve<FAST>(c, st, -g, zero, zero, 1.0, st.ds, st.dz);
     // cotangents; dz holds dlam_tilde, dlam = dlam_tilde / lam
     double* lamv = c.vb;        // lam = z
     double* dlamv = c.vb + mi;  // dlam = dlam_tilde / lam
