@@ -651,42 +651,87 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       off[I] = cl < n ? cl : n - 1;
       msk[I] = cl < n ? 1.0 : 0.0;
     }
-    auto fetch = [&](int k0, double (&g)[NTI], double& dk, double& zk) {
-      const int k = k0 + rq;
-      const int kc = k < mi ? k : mi - 1;
-      dk = k < mi ? c.vb[kc] : 0.0;
-      zk = (RES && k < mi) ? vz[kc] : 0.0;
+    // raw loads of a step's G row entries (input precision), D and z; converted later
+    auto load_raw = [&](int k0, TS (&g)[NTI], double& dk, double& zk) {
+      const int kc = k0 + rq < mi ? k0 + rq : mi - 1;
+      dk = c.vb[kc];
+      zk = RES ? vz[kc] : 0.0;
       const TS* grow = c.Gs + kc * n;
 #pragma unroll
-      for (int I = 0; I < NTI; ++I) g[I] = (double)grow[off[I]] * msk[I];
+      for (int I = 0; I < NTI; ++I) g[I] = grow[off[I]];
     };
-    double gc[NTI], dc, zc;
-    fetch(0, gc, dc, zc);
+    double gc[NTI], zc, ac[NTI];
+    {
+      TS g0[NTI];
+      double d0, z0;
+      load_raw(0, g0, d0, z0);
+      const bool live = rq < mi;
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) {
+        gc[I] = (double)g0[I] * msk[I];
+        ac[I] = gc[I] * (live ? d0 : 0.0);
+      }
+      zc = live ? z0 : 0.0;
+    }
     for (int k0 = 0; k0 < mi; k0 += 4) {
-      double gn[NTI], dn, zn;
-      fetch(k0 + 4 < mi ? k0 + 4 : k0, gn, dn, zn);
+      const int kn = k0 + 4 < mi ? k0 + 4 : k0;
+      TS gr[NTI];
+      double dr, zr;
+      load_raw(kn, gr, dr, zr);
+      const bool live_n = kn + rq < mi;
+      double gx = 0.0;
+      // vector work of step g, issued behind the g-th MFMA: this step's G x row sums and G^T z
+      // share, then the next step's operands
+      auto task = [&](int g) {
+        if (RES) {
+          if (g == 0) {
+#pragma unroll
+            for (int I = 0; I < NTI; ++I) gx += gc[I] * xc[I];
+          }
+          if (g == 1) gx += dpp<0xB1>(gx);
+          if (g == 2) gx += dpp<0x4E>(gx);
+          if (g == 3) gx += dpp<0x141>(gx);
+          if (g == 4) {
+            gx += dpp<0x140>(gx);
+            const int k = k0 + rq;
+            if (cq == 0 && k < mi) c.vb[k] = gx;
+          }
+          if (g == 5) {
+#pragma unroll
+            for (int I = 0; I < NTI; ++I) zacc[I] += gc[I] * zc;
+          }
+        }
+        if (g == 6) {
+#pragma unroll
+          for (int I = 0; I < NTI; ++I) gc[I] = (double)gr[I] * msk[I];
+        }
+        if (g == 7) {
+          const double dn = live_n ? dr : 0.0;
+#pragma unroll
+          for (int I = 0; I < NTI; ++I) ac[I] = gc[I] * dn;
+          zc = live_n ? zr : 0.0;
+        }
+      };
+      // the MFMAs read this step's operands before the tasks replace them: keep copies
+      double am[NTI], bm[NTI];
+#pragma unroll
+      for (int I = 0; I < NTI; ++I) {
+        am[I] = ac[I];
+        bm[I] = gc[I];
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int J = 0; J < NTI; ++J) {
-        const double a = gc[J] * dc;
 #pragma unroll
-        for (int I = J; I < NTI; ++I)
-          T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, gc[I], T[tidx(I, J)], 0, 0, 0);
-      }
-      if (RES) {
-        double gx = 0.0;
-#pragma unroll
-        for (int I = 0; I < NTI; ++I) {
-          gx += gc[I] * xc[I];
-          zacc[I] += gc[I] * zc;
+        for (int I = J; I < NTI; ++I) {
+          T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[J], bm[I], T[tidx(I, J)], 0, 0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          task(J * NTI - J * (J - 1) / 2 + (I - J));
+          __builtin_amdgcn_sched_barrier(0);
         }
-        gx = row16_sum(gx);
-        const int k = k0 + rq;
-        if (cq == 0 && k < mi) c.vb[k] = gx;
       }
 #pragma unroll
-      for (int I = 0; I < NTI; ++I) gc[I] = gn[I];
-      dc = dn;
-      zc = zn;
+      for (int g = NTILE; g < 8; ++g) task(g);
     }
   }
   WSYNC();
