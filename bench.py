@@ -359,7 +359,7 @@ def main():
         "iters_relax_mean": float(it_r.mean()),
         "kernel_ms": {"fused_solve_relax": round(f_ms, 4), "fused_backward": round(b_ms, 4)},
         "roofline": {
-            "kernel": "qpx::wave::k_wave_solve<float,64,2> (solve + relax; float64 MFMA + VALU)",
+            "kernel": "qpx::wave::k_wave_solve<float,64,2,true> (solve + relax; float64 MFMA + VALU)",
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": PEAK_F64_TFLOPS,
