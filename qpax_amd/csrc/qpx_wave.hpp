@@ -912,25 +912,65 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
     }
   }
   pa = l < m ? pa * st.rdpiv : 0.0;
-  // backward: L^T x = y, column j of L^T is row j of L (rows clamped to m - 1 for the address;
-  // FAST selects between the row's compile-time offset and the last row's, one scalar select per
-  // row instead of the clamp's arithmetic)
-  const int tri_last = (m - 1) * m / 2;
-  auto col_of = [&](int j) {
-    if constexpr (FAST) return c.Lp[(j < m ? j * (j + 1) / 2 : tri_last) + l];
-    const int jj = j < m ? j : m - 1;
-    return c.Lp[jj * (jj + 1) / 2 + l];
-  };
-  {
+  // backward: L^T x = y, column j of L^T is row j of L (rows clamped to m - 1 for the address).
+  // FAST: a block of 16 rows that lies below m (one uniform test per block) reads its rows at
+  // compile-time offsets and needs only the l < j mask; the block that reaches past m keeps the
+  // clamp and the j < m test.
+  if constexpr (FAST) {
+    auto load_block = [&](int b, double (&v)[16]) {
+      const int jt = NMAX - 1 - 16 * b;
+      if (jt < m) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = c.Lp[(jt - u) * (jt - u + 1) / 2 + l];
+      } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = jt - u, jj = j < m ? j : m - 1;
+          v[u] = c.Lp[jj * (jj + 1) / 2 + l];
+        }
+      }
+    };
+    double lv[16];
+    load_block(0, lv);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      double nv[16];
+      if (b + 1 < NB) load_block(b + 1, nv);
+      const int jt = NMAX - 1 - 16 * b;
+      if (jt < m) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = jt - u;
+          if (j > 0) pa -= (l < j ? lv[u] : 0.0) * rl(pa, j);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = jt - u;
+          if (j > 0) pa -= (l < j && j < m ? lv[u] : 0.0) * rl(pa, j);
+        }
+      }
+      if (b + 1 < NB) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) lv[u] = nv[u];
+      }
+    }
+  } else {
     double lv[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) lv[u] = col_of(NMAX - 1 - u);
+    for (int u = 0; u < 16; ++u) {
+      const int j = NMAX - 1 - u, jj = j < m ? j : m - 1;
+      lv[u] = c.Lp[jj * (jj + 1) / 2 + l];
+    }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       double nv[16];
       if (b + 1 < NB) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) nv[u] = col_of(NMAX - 1 - (16 * (b + 1) + u));
+        for (int u = 0; u < 16; ++u) {
+          const int j = NMAX - 1 - (16 * (b + 1) + u), jj = j < m ? j : m - 1;
+          nv[u] = c.Lp[jj * (jj + 1) / 2 + l];
+        }
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
