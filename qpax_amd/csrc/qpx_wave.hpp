@@ -138,13 +138,35 @@ struct Ctx {
   double* vb;     // m_ineq doubles (aliases pb + 128)
 };
 
-template <typename TS, int NMAX, int MIC>
+template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, const TS* Q,
                                      const TS* q, const TS* A, const TS* b, const TS* G, const TS* h) {
-  const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l;
+  const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l, me = c.me;
   TS* Gs = const_cast<TS*>(c.Gs);
-  // G into LDS as one flat copy, 8 loads in flight per lane
-  {
+  // G into LDS as one flat copy.  FAST: 16-byte vectors when source and destination allow it,
+  // 16 in flight per lane; otherwise 8 scalar loads in flight per lane.
+  typedef TS V __attribute__((ext_vector_type(16 / sizeof(TS))));
+  constexpr int VW = 16 / sizeof(TS);
+  const bool vec = FAST && ((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(Gs)) & 15) == 0;
+  if (vec) {
+    const int tot = mi * n, nv = tot / VW;
+    const V* Gv = reinterpret_cast<const V*>(G);
+    V* Gsv = reinterpret_cast<V*>(Gs);
+    for (int e0 = 0; e0 < nv; e0 += 16 * 64) {
+      V g[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int e = e0 + 64 * u + l;
+        if (e < nv) g[u] = Gv[e];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int e = e0 + 64 * u + l;
+        if (e < nv) Gsv[e] = g[u];
+      }
+    }
+    for (int e = nv * VW + l; e < tot; e += 64) Gs[e] = G[e];
+  } else {
     const int tot = mi * n;
     int e = l;
     for (; e + 7 * 64 < tot; e += 8 * 64) {
@@ -176,12 +198,54 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
       for (int j = l; j < n; j += 64) Gs[k * n + j] = (TS)0 * Gs[k * n + j];
     }
   }
-  // K's constant part, 8 columns per trip with their loads in flight
-  if (l < m) {
-    for (int j0 = 0; j0 < m; j0 += 8) {
-      double v[8];
+  // K's constant part.  FAST: every lane issues the four candidate loads of each element at
+  // clamped (always valid) addresses -- Q[l][j], Q[j][l], A[j - np][l], A[l - np][j] -- for 16
+  // columns at a time, then selects, so no load waits behind a lane-divergent branch.
+  if constexpr (FAST) {
+    if (l < m) {
+      constexpr int CB = 16;
+      const int lq = l < n ? l : n - 1;
+      const int la = me > 0 ? (l < np ? 0 : (l - np < me ? l - np : me - 1)) : 0;
+      for (int j0 = 0; j0 < m; j0 += CB) {
+        TS q1[CB], q2[CB], a1[CB], a2[CB];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < CB; ++u) {
+          const int j = j0 + u, jq = j < n ? j : n - 1;
+          q1[u] = Q[lq * n + jq];
+          q2[u] = Q[jq * n + lq];
+        }
+        if (me > 0) {
+#pragma unroll
+          for (int u = 0; u < CB; ++u) {
+            const int j = j0 + u, jq = j < n ? j : n - 1;
+            const int ja = j < np ? 0 : (j - np < me ? j - np : me - 1);
+            a1[u] = A[ja * n + lq];
+            a2[u] = A[la * n + jq];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+          const int j = j0 + u;
+          if (j >= m) break;
+          double val;
+          if (l < n) {
+            val = j < n ? 0.5 * ((double)q1[u] + (double)q2[u]) : (j >= np && me > 0 ? (double)a1[u] : 0.0);
+          } else if (l < np) {
+            val = j == l ? 1.0 : 0.0;
+          } else {
+            val = j < n ? (double)a2[u] : 0.0;
+          }
+          c.KQ[j * m + l] = val;
+        }
+      }
+    }
+  } else {
+  constexpr int CB = 8;
+  if (l < m) {
+    for (int j0 = 0; j0 < m; j0 += CB) {
+      double v[CB];
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
         const int j = j0 + u;
         double val = 0.0;
         if (j < m) {
@@ -197,9 +261,10 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
         v[u] = val;
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < CB; ++u)
         if (j0 + u < m) c.KQ[(j0 + u) * m + l] = v[u];
     }
+  }
   }
   __threadfence_block();
   st.qa = l < n ? (double)q[l] : ((l >= np && l < m) ? -(double)b[l - np] : 0.0);
@@ -1173,7 +1238,7 @@ __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) 
     const int n = pi.n, me = pi.me, mi = pi.mi;
     const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + pi.ows, a.lds_per_wave);
     St<TS, NMAX, MIC> st;
-    load(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
+    load<FRES>(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
     init<FRES>(c, st);
     int conv = 0, it = 0;
     while (it < kMaxIters && conv == 0) {
@@ -1209,7 +1274,7 @@ __global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
     const auto c = make_ctx<TS, NMAX, MIC>(n, me, mi, a.ws + pi.ows, a.lds_per_wave);
     const int l = c.l, np = c.np, m = c.m;
     St<TS, NMAX, MIC> st;
-    load(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
+    load<FAST>(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
     st.xa = l < n ? (double)a.xr_in[pi.oq + l] : ((l >= np && l < m) ? (double)a.yr_in[pi.ob + (l - np)] : 0.0);
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) {
