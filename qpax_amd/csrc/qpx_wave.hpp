@@ -106,6 +106,31 @@ __device__ __forceinline__ double grow_dot(const TS* __restrict__ Gs, int n, int
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
+// grow_dot for even n: the row starts on a two-element boundary, so entries load in pairs.
+template <typename TS>
+__device__ __forceinline__ double grow_dot2(const TS* __restrict__ Gs, int n, int row, const double* __restrict__ v) {
+  typedef TS V2 __attribute__((ext_vector_type(2)));
+  const V2* g = reinterpret_cast<const V2*>(Gs + row * n);
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  const int n2 = n >> 1;
+  int j = 0;
+  for (; j + 4 <= n2; j += 4) {
+    V2 gg[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) gg[u] = g[j + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 vv = *reinterpret_cast<const double2*>(v + 2 * (j + u));
+      a[u] += (double)gg[u].x * vv.x + (double)gg[u].y * vv.y;
+    }
+  }
+  for (; j < n2; ++j) {
+    const V2 gg = g[j];
+    a[0] += (double)gg.x * v[2 * j] + (double)gg.y * v[2 * j + 1];
+  }
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
 // 1/d to full double precision: hardware reciprocal plus two Newton steps.
 __device__ __forceinline__ double recip(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -1055,7 +1080,7 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if (k < mi) {
-      const double g = grow_dot(c.Gs, n, k, c.col);
+      const double g = (FAST && (n & 1) == 0) ? grow_dot2(c.Gs, n, k, c.col) : grow_dot(c.Gs, n, k, c.col);
       const double v3 = sg * a3[cc], v2 = sg * a2[cc];
       const double dsk = v3 - g;
       ods[cc] = dsk;
