@@ -911,7 +911,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
         const double dj = V[t][t];
         const double pe = (sgn * dj > thr[t]) ? dj : sgn * kHugePivot;
         inv[t] = recip(pe);
-        if (l == cc) st.rdpiv = inv[t];
+        st.rdpiv = l == cc ? inv[t] : st.rdpiv;
 #pragma unroll
         for (int u = t + 1; u < 4; ++u) {
           lm[u][t] = V[u][t] * inv[t];
@@ -924,14 +924,20 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       const double i20 = -lm[2][0] - lm[2][1] * i10;
       const double i31 = -lm[3][1] - lm[3][2] * i21;
       const double i30 = -lm[3][0] - lm[3][1] * i10 - lm[3][2] * i20;
-      const double k0 = rq == 1 ? i10 : rq == 2 ? i20 : rq == 3 ? i30 : 0.0;
-      const double k1 = rq == 2 ? i21 : rq == 3 ? i31 : 0.0;
-      const double k2 = rq == 3 ? i32 : 0.0;
+      // per lane group rq, selected on the bits of rq (selects, not branches)
+      const bool b0 = rq & 1, b1 = rq & 2;
+      auto sel4 = [&](double v0, double v1, double v2, double v3) {
+        const double lo = b0 ? v1 : v0, hi = b0 ? v3 : v2;
+        return b1 ? hi : lo;
+      };
+      const double k0 = sel4(0.0, i10, i20, i30);
+      const double k1 = sel4(0.0, 0.0, i21, i31);
+      const double k2 = b0 && b1 ? i32 : 0.0;
 #pragma unroll
       for (int I = 0; I < NTI; ++I)
         if (I >= J) P[I] += k0 * R[0][I] + k1 * R[1][I] + k2 * R[2][I];
       // this lane group's pivot reciprocal; multipliers L[16I + cq][c0 + rq] into the packed factor
-      const double myinv = rq == 0 ? inv[0] : rq == 1 ? inv[1] : rq == 2 ? inv[2] : inv[3];
+      const double myinv = sel4(inv[0], inv[1], inv[2], inv[3]);
 #pragma unroll
       for (int I = 0; I < NTI; ++I) {
         if (I < J) continue;
