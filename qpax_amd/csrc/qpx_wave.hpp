@@ -1085,27 +1085,44 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
-    if (k < mi) {
-      const double g = (FAST && (n & 1) == 0) ? grow_dot2(c.Gs, n, k, c.col) : grow_dot(c.Gs, n, k, c.col);
+    if constexpr (FAST) {
+      // every lane forms a row product (rows past m_ineq clamped to the last) and selects
+      const int kc = k < mi ? k : mi - 1;
+      const double g = (n & 1) == 0 ? grow_dot2(c.Gs, n, kc, c.col) : grow_dot(c.Gs, n, kc, c.col);
       const double v3 = sg * a3[cc], v2 = sg * a2[cc];
       const double dsk = v3 - g;
-      ods[cc] = dsk;
-      odz[cc] = (v2 - st.z[cc] * dsk) / st.s[cc];
+      const double dzk = (v2 - st.z[cc] * dsk) / st.s[cc];
+      ods[cc] = k < mi ? dsk : 0.0;
+      odz[cc] = k < mi ? dzk : 0.0;
     } else {
-      ods[cc] = 0.0;
-      odz[cc] = 0.0;
+      if (k < mi) {
+        const double g = grow_dot(c.Gs, n, k, c.col);
+        const double v3 = sg * a3[cc], v2 = sg * a2[cc];
+        const double dsk = v3 - g;
+        ods[cc] = dsk;
+        odz[cc] = (v2 - st.z[cc] * dsk) / st.s[cc];
+      } else {
+        ods[cc] = 0.0;
+        odz[cc] = 0.0;
+      }
     }
   }
   WSYNC();
 }
 
-template <int MIC>
+// FAST: the ratio is formed in every lane and selected (no divergent branch around the division).
+template <bool FAST, int MIC>
 __device__ __forceinline__ double ort(const double (&x)[MIC], const double (&dx)[MIC], int mi, int l) {
   double a = 1.0;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
-    if (k < mi && dx[cc] < 0) a = nmin(a, -x[cc] / dx[cc]);
+    if constexpr (FAST) {
+      const double r = -x[cc] / dx[cc];
+      a = (k < mi && dx[cc] < 0) ? nmin(a, r) : a;
+    } else {
+      if (k < mi && dx[cc] < 0) a = nmin(a, -x[cc] / dx[cc]);
+    }
   }
   return wmin(a);
 }
@@ -1142,10 +1159,10 @@ __device__ __forceinline__ void init(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
   }
 }
 
-template <typename TS, int NMAX, int MIC>
+template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void step(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, bool relax_rule) {
-  const double as = ort<MIC>(st.s, st.ds, c.mi, c.l);
-  const double az = ort<MIC>(st.z, st.dz, c.mi, c.l);
+  const double as = ort<FAST, MIC>(st.s, st.ds, c.mi, c.l);
+  const double az = ort<FAST, MIC>(st.z, st.dz, c.mi, c.l);
   const double alpha = relax_rule ? 0.99 * nmin(nmin(1.0, 0.99 * as), 0.99 * az) : 0.99 * nmin(as, az);
   st.xa += alpha * st.da;
 #pragma unroll
@@ -1173,7 +1190,7 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
   for (int cc = 0; cc < MIC; ++cc) if (64 * cc + l < mi) sz += st.s[cc] * st.z[cc];
   sz = wsum(sz);
   const double mu = sz / mi;
-  const double a = nmin(ort<MIC>(st.s, st.dsa, mi, l), ort<MIC>(st.z, st.dza, mi, l));
+  const double a = nmin(ort<FRES, MIC>(st.s, st.dsa, mi, l), ort<FRES, MIC>(st.z, st.dza, mi, l));
   double gg = 0.0;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc)
@@ -1184,7 +1201,7 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) st.r2[cc] = st.r2[cc] - (sm - st.dsa[cc] * st.dza[cc]);
   solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
-  step(c, st, false);
+  step<FRES>(c, st, false);
   return nrm < tol ? 1 : 0;
 }
 
@@ -1200,7 +1217,7 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
     factor<false>(c, st, false, 0.0);
   }
   solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
-  step(c, st, true);
+  step<FRES>(c, st, true);
   return nrm < tol ? 1 : 0;
 }
 
