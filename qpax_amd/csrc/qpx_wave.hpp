@@ -139,6 +139,15 @@ __device__ __forceinline__ double recip(double d) {
   return r;
 }
 
+// 1/d with a shorter dependency chain: the hardware reciprocal r0 is accurate to 2^29 ulp
+// (|e| <= 2^-23 with e = 1 - d r0), and 1/d = r0 (1 + e + e^2 + ...), so r0 + r0 (e + e^2) leaves
+// a truncation of e^3 <= 2^-69 under the rounding of the three fused operations.
+__device__ __forceinline__ double recip_c(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  const double e = fma(-d, r, 1.0);
+  return fma(r, fma(e, e, e), r);
+}
+
 
 template <typename TS, int NMAX, int MIC>
 struct St {
@@ -890,34 +899,45 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       double P[NTI];
 #pragma unroll
       for (int I = 0; I < NTI; ++I) P[I] = I >= J ? T[tidx(I, J)][pp] : 0.0;
-      // the panel's 4 x 4 diagonal block as uniform values, and the broadcasts of its first three
-      // rows to every lane group (both depend on the panel's input only)
+      // the upper triangle of the panel's 4 x 4 diagonal block as uniform values (the block is
+      // symmetric), and the broadcasts of its first three rows to every lane group (both depend on
+      // the panel's input only)
       double V[4][4];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) V[i][u] = rl(P[J], 16 * i + jc + u);
+        for (int u = i; u < 4; ++u) V[i][u] = rl(P[J], 16 * i + jc + u);
       double R[3][NTI];
 #pragma unroll
       for (int t = 0; t < 3; ++t)
 #pragma unroll
         for (int I = 0; I < NTI; ++I) R[t][I] = I >= J ? __shfl(P[I], 16 * t + cq, 64) : 0.0;
-      // eliminate the block: pivots, and multipliers lm[u][t] of the unit-lower block factor
+      // the broadcasts go out before the pivot chain, whose latency then covers theirs
+      __builtin_amdgcn_sched_barrier(0);
+      // eliminate the block (L D L^T on its upper triangle): pivots, and multipliers
+      // lm[u][t] = U[t][u] / D[t] of the unit-lower block factor.  The chain from one pivot to the
+      // next is kept short: the reciprocal is formed before the pivot test (a failed test selects
+      // 1 / (+-kHugePivot) afterwards), and the next pivot is one fused operation on U[t][t+1]^2,
+      // formed before 1 / pivot t is known.
       double inv[4], lm[4][4];
+      double dj = V[0][0];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int cc = c0 + t;
         const double sgn = cc < np ? 1.0 : -1.0;
-        const double dj = V[t][t];
-        const double pe = (sgn * dj > thr[t]) ? dj : sgn * kHugePivot;
-        inv[t] = recip(pe);
+        const bool ok = sgn * dj > thr[t];
+        const double r = recip_c(dj);
+        inv[t] = ok ? r : sgn * (1.0 / kHugePivot);
         st.rdpiv = l == cc ? inv[t] : st.rdpiv;
+        if (t == 3) break;
+        dj = fma(-(V[t][t + 1] * V[t][t + 1]), inv[t], V[t + 1][t + 1]);
 #pragma unroll
-        for (int u = t + 1; u < 4; ++u) {
-          lm[u][t] = V[u][t] * inv[t];
+        for (int u = t + 1; u < 4; ++u) lm[u][t] = V[t][u] * inv[t];
 #pragma unroll
-          for (int w = t + 1; w < 4; ++w) V[u][w] -= lm[u][t] * V[t][w];
-        }
+        for (int u = t + 1; u < 4; ++u)
+#pragma unroll
+          for (int w = u; w < 4; ++w)
+            if (w != t + 1) V[u][w] -= lm[u][t] * V[t][w];
       }
       // pivot rows U = L_block^{-1} R: row u gains sum_{t<u} Linv[u][t] * (original row t)
       const double i10 = -lm[1][0], i21 = -lm[2][1], i32 = -lm[3][2];
@@ -935,7 +955,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       const double k2 = b0 && b1 ? i32 : 0.0;
 #pragma unroll
       for (int I = 0; I < NTI; ++I)
-        if (I >= J) P[I] += k0 * R[0][I] + k1 * R[1][I] + k2 * R[2][I];
+        if (I >= J) P[I] = fma(k0, R[0][I], fma(k1, R[1][I], fma(k2, R[2][I], P[I])));
       // this lane group's pivot reciprocal; multipliers L[16I + cq][c0 + rq] into the packed factor
       const double myinv = sel4(inv[0], inv[1], inv[2], inv[3]);
 #pragma unroll
