@@ -139,6 +139,11 @@ __device__ __forceinline__ double recip(double d) {
   return r;
 }
 
+template <bool B>
+struct BoolTag {
+  static constexpr bool value = B;
+};
+
 // 1/d with a shorter dependency chain: the hardware reciprocal r0 is accurate to 2^29 ulp
 // (|e| <= 2^-23 with e = 1 - d r0), and 1/d = r0 (1 + e + e^2 + ...), so r0 + r0 (e + e^2) leaves
 // a truncation of e^3 <= 2^-69 under the rounding of the three fused operations.
@@ -740,7 +745,12 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     }
   }
   // G^T D G into the tiles (operands as in factor, A from the tile row, B from the tile column),
-  // with G x and G^T z under the MFMAs when RES
+  // with G x and G^T z under the MFMAs when RES.  When G reaches only 1..4 columns of the last tile
+  // column (n = 50 in a 64-wide K: columns 48, 49), that tile column takes its products from
+  // 4x4x4 MFMAs over a 4-column strip, at a quarter of the cost of the 16x16x4 tiles.  In the 4x4x4
+  // layout, block b (lanes with (l & 15) >> 2 == b) holds A[i][k] in lane 16k + 4b + i, B[k][j] in
+  // lane 16k + 4b + j and D[i][j] in lane 16i + 4b + j: the 16x16x4 A operand of a tile row is
+  // then the A operand of its 16 rows in four blocks, and every block takes the strip's B.
   {
     int off[NTI];
     double msk[NTI];
@@ -750,87 +760,126 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       off[I] = cl < n ? cl : n - 1;
       msk[I] = cl < n ? 1.0 : 0.0;
     }
+    const int rem = n - 16 * (NTI - 1);
+    const bool strip = NTI > 1 && rem >= 1 && rem <= 4;
+    const int cs = 16 * (NTI - 1) + (cq & 3);
+    const int offs = cs < n ? cs : n - 1;
+    const double msks = cs < n ? 1.0 : 0.0;
     // raw loads of a step's G row entries (input precision), D and z; converted later
-    auto load_raw = [&](int k0, TS (&g)[NTI], double& dk, double& zk) {
+    auto load_raw = [&](int k0, TS (&g)[NTI], TS& gs, double& dk, double& zk, bool with_strip) {
       const int kc = k0 + rq < mi ? k0 + rq : mi - 1;
       dk = c.vb[kc];
       zk = RES ? vz[kc] : 0.0;
       const TS* grow = c.Gs + kc * n;
 #pragma unroll
       for (int I = 0; I < NTI; ++I) g[I] = grow[off[I]];
+      gs = with_strip ? grow[offs] : (TS)0;
     };
-    double gc[NTI], zc, ac[NTI];
-    {
-      TS g0[NTI];
-      double d0, z0;
-      load_raw(0, g0, d0, z0);
-      const bool live = rq < mi;
+    auto gtdg = [&](auto tag) {
+      constexpr bool S = decltype(tag)::value;
+      double gc[NTI], zc, ac[NTI], gsc = 0.0, sacc[NTI];
 #pragma unroll
-      for (int I = 0; I < NTI; ++I) {
-        gc[I] = (double)g0[I] * msk[I];
-        ac[I] = gc[I] * (live ? d0 : 0.0);
+      for (int J = 0; J < NTI; ++J) sacc[J] = 0.0;
+      {
+        TS g0[NTI], gs0;
+        double d0, z0;
+        load_raw(0, g0, gs0, d0, z0, S);
+        const bool live = rq < mi;
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) {
+          gc[I] = (double)g0[I] * msk[I];
+          ac[I] = gc[I] * (live ? d0 : 0.0);
+        }
+        if (S) gsc = (double)gs0 * msks;
+        zc = live ? z0 : 0.0;
       }
-      zc = live ? z0 : 0.0;
-    }
-    for (int k0 = 0; k0 < mi; k0 += 4) {
-      const int kn = k0 + 4 < mi ? k0 + 4 : k0;
-      TS gr[NTI];
-      double dr, zr;
-      load_raw(kn, gr, dr, zr);
-      const bool live_n = kn + rq < mi;
-      double gx = 0.0;
-      // vector work of step g, issued behind the g-th MFMA: this step's G x row sums and G^T z
-      // share, then the next step's operands
-      auto task = [&](int g) {
-        if (RES) {
-          if (g == 0) {
+      for (int k0 = 0; k0 < mi; k0 += 4) {
+        const int kn = k0 + 4 < mi ? k0 + 4 : k0;
+        TS gr[NTI], gsr;
+        double dr, zr;
+        load_raw(kn, gr, gsr, dr, zr, S);
+        const bool live_n = kn + rq < mi;
+        double gx = 0.0;
+        // vector work of step g, issued behind the g-th MFMA: this step's G x row sums and G^T z
+        // share, then the next step's operands
+        auto task = [&](int g) {
+          if (RES) {
+            if (g == 0) {
 #pragma unroll
-            for (int I = 0; I < NTI; ++I) gx += gc[I] * xc[I];
-          }
-          if (g == 1) gx += dpp<0xB1>(gx);
-          if (g == 2) gx += dpp<0x4E>(gx);
-          if (g == 3) gx += dpp<0x141>(gx);
-          if (g == 4) {
-            gx += dpp<0x140>(gx);
-            const int k = k0 + rq;
-            if (cq == 0 && k < mi) c.vb[k] = gx;
-          }
-          if (g == 5) {
+              for (int I = 0; I < NTI; ++I) gx += gc[I] * xc[I];
+            }
+            if (g == 1) gx += dpp<0xB1>(gx);
+            if (g == 2) gx += dpp<0x4E>(gx);
+            if (g == 3) gx += dpp<0x141>(gx);
+            if (g == 4) {
+              gx += dpp<0x140>(gx);
+              const int k = k0 + rq;
+              if (cq == 0 && k < mi) c.vb[k] = gx;
+            }
+            if (g == 5) {
 #pragma unroll
-            for (int I = 0; I < NTI; ++I) zacc[I] += gc[I] * zc;
+              for (int I = 0; I < NTI; ++I) zacc[I] += gc[I] * zc;
+            }
+          }
+          if (g == 6) {
+#pragma unroll
+            for (int I = 0; I < NTI; ++I) gc[I] = (double)gr[I] * msk[I];
+            if (S) gsc = (double)gsr * msks;
+          }
+          if (g == 7) {
+            const double dn = live_n ? dr : 0.0;
+#pragma unroll
+            for (int I = 0; I < NTI; ++I) ac[I] = gc[I] * dn;
+            zc = live_n ? zr : 0.0;
+          }
+        };
+        // the MFMAs read this step's operands before the tasks replace them: keep copies
+        double am[NTI], bm[NTI];
+#pragma unroll
+        for (int I = 0; I < NTI; ++I) {
+          am[I] = ac[I];
+          bm[I] = gc[I];
+        }
+        const double bs = gsc;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int J = 0; J < NTI; ++J) {
+#pragma unroll
+          for (int I = J; I < NTI; ++I) {
+            if (S && I == NTI - 1)
+              sacc[J] = __builtin_amdgcn_mfma_f64_4x4x4f64(am[J], bs, sacc[J], 0, 0, 0);
+            else
+              T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[J], bm[I], T[tidx(I, J)], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            task(J * NTI - J * (J - 1) / 2 + (I - J));
+            __builtin_amdgcn_sched_barrier(0);
           }
         }
-        if (g == 6) {
 #pragma unroll
-          for (int I = 0; I < NTI; ++I) gc[I] = (double)gr[I] * msk[I];
-        }
-        if (g == 7) {
-          const double dn = live_n ? dr : 0.0;
-#pragma unroll
-          for (int I = 0; I < NTI; ++I) ac[I] = gc[I] * dn;
-          zc = live_n ? zr : 0.0;
-        }
-      };
-      // the MFMAs read this step's operands before the tasks replace them: keep copies
-      double am[NTI], bm[NTI];
-#pragma unroll
-      for (int I = 0; I < NTI; ++I) {
-        am[I] = ac[I];
-        bm[I] = gc[I];
+        for (int g = NTILE; g < 8; ++g) task(g);
       }
-      __builtin_amdgcn_sched_barrier(0);
+      if (S) {
+        // strip rows 16J + 4b + i, columns 16(NTI-1) + j (lane 16i + 4b + j) into register b of
+        // tile (J, NTI-1), lane 16i + j
 #pragma unroll
-      for (int J = 0; J < NTI; ++J) {
+        for (int J = 0; J < NTI; ++J) {
 #pragma unroll
-        for (int I = J; I < NTI; ++I) {
-          T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[J], bm[I], T[tidx(I, J)], 0, 0, 0);
-          __builtin_amdgcn_sched_barrier(0);
-          task(J * NTI - J * (J - 1) / 2 + (I - J));
-          __builtin_amdgcn_sched_barrier(0);
+          for (int r = 0; r < 4; ++r) {
+            const double v = r == 0 ? sacc[J] : __shfl(sacc[J], l + 4 * r, 64);
+            T[tidx(NTI - 1, J)][r] += cq < 4 ? v : 0.0;
+          }
         }
       }
-#pragma unroll
-      for (int g = NTILE; g < 8; ++g) task(g);
+    };
+    // (the strip's extra live registers cost more than it saves in the factorisations without the
+    // fused residuals: the backward kernel's and the first one)
+    if constexpr (RES) {
+      if (strip)
+        gtdg(BoolTag<true>{});
+      else
+        gtdg(BoolTag<false>{});
+    } else {
+      gtdg(BoolTag<false>{});
     }
   }
   WSYNC();
