@@ -1078,42 +1078,35 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   }
   pa = l < m ? pa * st.rdpiv : 0.0;
   // backward: L^T x = y, column j of L^T is row j of L (rows clamped to m - 1 for the address).
-  // FAST: a block of 16 rows that lies below m (one uniform test per block) reads its rows at
-  // compile-time offsets and needs only the l < j mask; the block that reaches past m keeps the
-  // clamp and the j < m test.
-  if constexpr (FAST) {
-    auto load_block = [&](int b, double (&v)[16]) {
-      const int jt = NMAX - 1 - 16 * b;
-      if (jt < m) {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = c.Lp[(jt - u) * (jt - u + 1) / 2 + l];
-      } else {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = jt - u, jj = j < m ? j : m - 1;
-          v[u] = c.Lp[jj * (jj + 1) / 2 + l];
-        }
-      }
-    };
+  // FAST: when the reads of rows m..NMAX-1 stay inside the panel buffer behind the factor (at
+  // least 512 doubles), every block reads its rows at compile-time offsets, branch-free, with the
+  // next block's loads issued a block ahead; the select drops what those rows hold.  Rows below
+  // kMinTail are below every m that passes the test and skip the j < m term.
+  constexpr int kTailReach = (NMAX - 1) * NMAX / 2 + 64 - 512;
+  constexpr int kMinTail = [] {
+    int mm = 1;
+    while (mm * (mm + 1) / 2 < kTailReach) ++mm;
+    return mm;
+  }();
+  if (FAST && m * (m + 1) / 2 >= kTailReach) {
     double lv[16];
-    load_block(0, lv);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lv[u] = c.Lp[(NMAX - 1 - u) * (NMAX - u) / 2 + l];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       double nv[16];
-      if (b + 1 < NB) load_block(b + 1, nv);
-      const int jt = NMAX - 1 - 16 * b;
-      if (jt < m) {
+      if (b + 1 < NB) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-          const int j = jt - u;
-          if (j > 0) pa -= (l < j ? lv[u] : 0.0) * rl(pa, j);
+          const int j = NMAX - 1 - 16 * (b + 1) - u;
+          nv[u] = c.Lp[j * (j + 1) / 2 + l];
         }
-      } else {
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = jt - u;
-          if (j > 0) pa -= (l < j && j < m ? lv[u] : 0.0) * rl(pa, j);
-        }
+      for (int u = 0; u < 16; ++u) {
+        const int j = NMAX - 1 - 16 * b - u;
+        if (j > 0) pa -= ((j < kMinTail || j < m) && l < j ? lv[u] : 0.0) * rl(pa, j);
       }
       if (b + 1 < NB) {
 #pragma unroll
