@@ -64,24 +64,45 @@ __device__ __forceinline__ double row16_sum(double v) {
 
 #define WSYNC() __builtin_amdgcn_wave_barrier()
 
-// sum_k G[k][col] * v[k] over k < mi (v broadcast from LDS); four rows per trip.
+// sum_k G[k][col] * v[k] over k < mi (v broadcast from LDS); eight rows per step, with two
+// buffers: the next step's loads are in flight while this step accumulates (a last step reloads
+// its own rows).
 template <typename TS>
 __device__ __forceinline__ double gcol_dot(const TS* __restrict__ Gs, int n, int mi, int col,
                                            const double* __restrict__ v) {
   double a[4] = {0.0, 0.0, 0.0, 0.0};
-  int k = 0;
-  for (; k + 8 <= mi; k += 8) {
-    double g[8];
+  const int nk = mi & ~7;
+  auto ld = [&](int k, TS (&g)[8], double2 (&w)[4]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) g[u] = (double)Gs[(k + u) * n + col];
+    for (int u = 0; u < 8; ++u) g[u] = Gs[(k + u) * n + col];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = *reinterpret_cast<const double2*>(v + k + 2 * u);
+  };
+  auto acc = [&](const TS (&g)[8], const double2 (&w)[4]) {
 #pragma unroll
     for (int u = 0; u < 8; u += 2) {
-      const double2 vv = *reinterpret_cast<const double2*>(v + k + u);
-      a[(u >> 1) & 3] += g[u] * vv.x;
-      a[(u >> 1) & 3] += g[u + 1] * vv.y;
+      a[(u >> 1) & 3] += (double)g[u] * w[u >> 1].x;
+      a[(u >> 1) & 3] += (double)g[u + 1] * w[u >> 1].y;
+    }
+  };
+  if (nk > 0) {
+    TS g0[8], g1[8];
+    double2 w0[4], w1[4];
+    ld(0, g0, w0);
+    for (int k = 0;;) {
+      ld(k + 8 < nk ? k + 8 : k, g1, w1);
+      __builtin_amdgcn_sched_barrier(0);
+      acc(g0, w0);
+      k += 8;
+      if (k >= nk) break;
+      ld(k + 8 < nk ? k + 8 : k, g0, w0);
+      __builtin_amdgcn_sched_barrier(0);
+      acc(g1, w1);
+      k += 8;
+      if (k >= nk) break;
     }
   }
-  for (; k < mi; ++k) a[0] += (double)Gs[k * n + col] * v[k];
+  for (int k = nk; k < mi; ++k) a[0] += (double)Gs[k * n + col] * v[k];
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
@@ -106,25 +127,44 @@ __device__ __forceinline__ double grow_dot(const TS* __restrict__ Gs, int n, int
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
-// grow_dot for even n: the row starts on a two-element boundary, so entries load in pairs.
+// grow_dot for even n: the row starts on a two-element boundary, so entries load in pairs; two
+// buffers as in gcol_dot.
 template <typename TS>
 __device__ __forceinline__ double grow_dot2(const TS* __restrict__ Gs, int n, int row, const double* __restrict__ v) {
   typedef TS V2 __attribute__((ext_vector_type(2)));
   const V2* g = reinterpret_cast<const V2*>(Gs + row * n);
   double a[4] = {0.0, 0.0, 0.0, 0.0};
   const int n2 = n >> 1;
-  int j = 0;
-  for (; j + 4 <= n2; j += 4) {
-    V2 gg[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) gg[u] = g[j + u];
+  const int nj = n2 & ~3;
+  auto ld = [&](int j, V2 (&gg)[4], double2 (&w)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double2 vv = *reinterpret_cast<const double2*>(v + 2 * (j + u));
-      a[u] += (double)gg[u].x * vv.x + (double)gg[u].y * vv.y;
+      gg[u] = g[j + u];
+      w[u] = *reinterpret_cast<const double2*>(v + 2 * (j + u));
+    }
+  };
+  auto acc = [&](const V2 (&gg)[4], const double2 (&w)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] += (double)gg[u].x * w[u].x + (double)gg[u].y * w[u].y;
+  };
+  if (nj > 0) {
+    V2 g0[4], g1[4];
+    double2 w0[4], w1[4];
+    ld(0, g0, w0);
+    for (int j = 0;;) {
+      ld(j + 4 < nj ? j + 4 : j, g1, w1);
+      __builtin_amdgcn_sched_barrier(0);
+      acc(g0, w0);
+      j += 4;
+      if (j >= nj) break;
+      ld(j + 4 < nj ? j + 4 : j, g0, w0);
+      __builtin_amdgcn_sched_barrier(0);
+      acc(g1, w1);
+      j += 4;
+      if (j >= nj) break;
     }
   }
-  for (; j < n2; ++j) {
+  for (int j = nj; j < n2; ++j) {
     const V2 gg = g[j];
     a[0] += (double)gg.x * v[2 * j] + (double)gg.y * v[2 * j + 1];
   }
