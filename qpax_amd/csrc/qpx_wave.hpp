@@ -840,7 +840,8 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
         load_raw(kn, gr, gsr, dr, zr, S);
         const bool live_n = kn + rq < mi;
         double gx = 0.0;
-        // vector work of step g, issued behind the g-th MFMA: this step's G x row sums and G^T z
+        // the step's vector work, issued after its MFMAs (float64 MFMAs occupy the SIMD's vector
+        // datapath, so nothing is hidden by interleaving them): this step's G x row sums and G^T z
         // share, then the next step's operands
         auto task = [&](int g) {
           if (RES) {
@@ -890,13 +891,11 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
               sacc[J] = __builtin_amdgcn_mfma_f64_4x4x4f64(am[J], bs, sacc[J], 0, 0, 0);
             else
               T[tidx(I, J)] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[J], bm[I], T[tidx(I, J)], 0, 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
-            task(J * NTI - J * (J - 1) / 2 + (I - J));
-            __builtin_amdgcn_sched_barrier(0);
           }
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int g = NTILE; g < 8; ++g) task(g);
+        for (int g = 0; g < (NTILE > 8 ? NTILE : 8); ++g) task(g);
       }
       if (S) {
         // strip rows 16J + 4b + i, columns 16(NTI-1) + j (lane 16i + 4b + j) into register b of
