@@ -467,6 +467,7 @@ int64_t stride_words(int n, int me, int mi) {
   const int64_t m = qpx::wave::padded_n(n) + me;
   int64_t w = qpx::elastic_words(n, me, mi);
   if (w < qpx::round_up(m * m, 64)) w = qpx::round_up(m * m, 64);
+  if (w < qpx::wave::split_words(n, me)) w = qpx::wave::split_words(n, me);
   const int64_t g = qpx::group::group_words(n, me, mi);
   return w > g ? w : g;
 }

@@ -219,7 +219,8 @@ struct Ctx {
 
 template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, const TS* Q,
-                                     const TS* q, const TS* A, const TS* b, const TS* G, const TS* h) {
+                                     const TS* q, const TS* A, const TS* b, const TS* G, const TS* h,
+                                     bool build_kq = true) {
   const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l, me = c.me;
   TS* Gs = const_cast<TS*>(c.Gs);
   // G into LDS as one flat copy.  FAST: 16-byte vectors when source and destination allow it,
@@ -281,7 +282,7 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
   // clamped (always valid) addresses -- Q[l][j], Q[j][l], A[j - np][l], A[l - np][j] -- for 16
   // columns at a time, then selects, so no load waits behind a lane-divergent branch.
   if constexpr (FAST) {
-    if (l < m) {
+    if (l < m && build_kq) {
       constexpr int CB = 16;
       const int lq = l < n ? l : n - 1;
       const int la = me > 0 ? (l < np ? 0 : (l - np < me ? l - np : me - 1)) : 0;
@@ -1379,8 +1380,96 @@ __device__ __forceinline__ int64_t next_problem(const Args<TS>& a, int& round) {
 // Forward kernel: solve (and optionally relax) QPs, one per wavefront at a time.  FRES selects
 // factor_up with the residuals formed inside its G^T D G pass (uniform batches) over the separate
 // residual pass and factor (ragged batches).
+// Forward launch of a uniform batch with the relaxations as work items of their own: tickets
+// 0..B-1 are the PC solves, B..2B-1 the relaxations.  A solve leaves its post-processed float64
+// state (s = inf, z = 0 on rows with infinite h) after the constant matrix in its workspace slice
+// and raises a ready flag (release); the relaxation of the same problem waits for the flag
+// (acquire), reloads G into LDS and continues from that state.  A relaxation's ticket is taken
+// after its solve's, which a running wavefront holds, so the wait ends; the wait is bounded all
+// the same (after kSplitSpin polls the relaxation reports riters = -1 and stops).  The last work
+// items of the launch are then relaxations (about 4 iterations) instead of whole problems.
+constexpr int kSplitSpin = 1 << 22;
+
+template <typename TS, int NMAX, int MIC>
+__device__ __forceinline__ void wave_split(const Args<TS>& a) {
+  unsigned tk = 0;
+  if ((threadIdx.x & 63) == 0) tk = atomicAdd(a.qhead, 1u);
+  const int64_t u = (int64_t)__builtin_amdgcn_readfirstlane(tk);
+  if (u >= 2 * a.B) return;
+  const bool rel = u >= a.B;
+  const int64_t p = rel ? u - a.B : u;
+  const PIdx pi = pidx(a, p);
+  const auto c = make_ctx<TS, NMAX, MIC>(pi.n, pi.me, pi.mi, a.ws + pi.ows, a.lds_per_wave);
+  double* sw = a.ws + pi.ows + split_state_off(pi.n, pi.me);
+  unsigned* ready = reinterpret_cast<unsigned*>(sw + kSplitStateWords);
+  const int l = c.l;
+  if (rel) {
+    bool ok = false;
+    for (int t = 0; t < kSplitSpin; ++t) {
+      if (__hip_atomic_load(ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        ok = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+    if (!ok) {
+      if (l == 0) { a.rconv[p] = 0; a.riters[p] = -1; }
+      return;
+    }
+  }
+  St<TS, NMAX, MIC> st;
+  load<true>(c, st, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh, !rel);
+  if (!rel) {
+    init<true>(c, st);
+    int conv = 0, it = 0;
+    while (it < kMaxIters && conv == 0) {
+      conv = pc_iter<true>(c, st, a.tol);
+      ++it;
+    }
+    store_state(c, st, pi, a.x, a.s, a.z, a.y, true);
+    if (l == 0) { a.conv[p] = conv; a.iters[p] = it; }
+    sw[l] = st.xa;
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      sw[64 + 128 * cc + l] = st.hinf[cc] ? INFINITY : st.s[cc];
+      sw[128 + 128 * cc + l] = st.hinf[cc] ? 0.0 : st.z[cc];
+    }
+    __threadfence();
+    if (l == 0) __hip_atomic_store(ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    st.xa = sw[l];
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      st.s[cc] = sw[64 + 128 * cc + l];
+      st.z[cc] = sw[128 + 128 * cc + l];
+    }
+    int rconv = 0, rit = 0;
+    while (rit < kMaxIters && rconv == 0) {
+      rconv = relax_iter<true>(c, st, a.tol, a.kappa);
+      ++rit;
+    }
+    store_state(c, st, pi, a.xr, a.sr, a.zr, a.yr, false);
+    if (l == 0) { a.rconv[p] = rconv; a.riters[p] = rit; }
+  }
+}
+
+// Resets the ticket counter and the ready flags of a split forward launch (a kernel, so that a
+// captured graph replays it).
+template <typename TS>
+__global__ __launch_bounds__(256) void k_wave_split_clear(Args<TS> a, int64_t flag_word) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) a.qhead[0] = 0u;
+  if (p < a.B) *reinterpret_cast<unsigned*>(a.ws + p * a.stride + flag_word) = 0u;
+}
+
 template <typename TS, int NMAX, int MIC, bool FRES>
 __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) {
+  if constexpr (FRES) {
+    if (with_relax == 2) {
+      wave_split<TS, NMAX, MIC>(a);
+      return;
+    }
+  }
   int round = 0;
   for (int64_t p; (p = next_problem(a, round)) >= 0;) {
     const PIdx pi = pidx(a, p);

@@ -29,6 +29,16 @@ __host__ __device__ inline Lds lds_layout(int n, int me, int mi, int ts) {
   return w;
 }
 
+// Split forward launches (uniform batches): per problem, the float64 state [x; y] (64 lanes), s
+// and z (64 x MIC each, MIC <= 4) and a ready flag follow the m x m constant matrix in the
+// problem's workspace slice.
+__host__ __device__ inline int64_t split_state_off(int n, int me) {
+  const int64_t m = padded_n(n) + me;
+  return (m * m + 63) / 64 * 64;
+}
+constexpr int kSplitStateWords = 64 + 2 * 64 * 4;
+__host__ __device__ inline int64_t split_words(int n, int me) { return split_state_off(n, me) + kSplitStateWords + 64; }
+
 template <typename TS, int NMAX, int MIC>
 hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st, int relax);
 template <typename TS, int NMAX, int MIC>
