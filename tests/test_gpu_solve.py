@@ -67,3 +67,43 @@ def test_plain_matches_oracle(name, gen, dt, tol, xtol):
     np.testing.assert_array_equal(conv, co)
     assert np.abs(iters - io).max() <= 1
     assert rel_err(x[ok], xo[ok]).max() < xtol
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_relaxed_graph_replays_match_eager(dt):
+    """A uniform solve + relax runs its PC solves and relaxations as separate work items (ready
+    flags and a ticket counter in the workspace, reset by a kernel at each launch): captured in a
+    hipGraph and replayed, with unrelated work in between, every replay gives the eager result
+    bit for bit, and that result is the oracle's."""
+    from qpax_amd._tensors import normalize
+    from qpax_amd.pdip import solve_qp_relaxed_batch
+
+    data = P.random_feasible(256, 50, 10, 100, seed=21)
+    print(f"[case] relaxed graph replay dt={dt} B=256 n=50", flush=True)
+    bt = normalize(*[torch.tensor(np.asarray(v), dtype=dt, device="cuda") for v in data])
+    eager = [t.clone() for t in solve_qp_relaxed_batch(bt, 1e-5, 1e-3)]
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        solve_qp_relaxed_batch(bt, 1e-5, 1e-3)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = solve_qp_relaxed_batch(bt, 1e-5, 1e-3)
+    for rep in range(3):
+        for t in out:
+            t.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(out, eager):
+            assert torch.equal(a, b), rep
+        (torch.randn(1024, device="cuda") * 2).sum().item()
+    npdt = np.float32 if dt == torch.float32 else np.float64
+    d64 = [np.asarray(np.asarray(v, npdt), np.float64) for v in data]
+    xo, rel, info = O.solve_qp_primal_fwd(*d64, solver_tol=1e-5, target_kappa=1e-3)
+    ok = (info[0] == 1) & (info[2] == 1)
+    assert ok.mean() > 0.9
+    xr = to_np(eager[6]).reshape(256, 50).astype(np.float64)
+    assert rel_err(xr[ok], rel[0][ok]).max() < (1e-7 if dt == torch.float64 else 1e-4)
