@@ -202,6 +202,9 @@ struct St {
   double rdpiv;   // 1 / pivot of this lane's row
   double xa, ra, da;
   double s[MIC], z[MIC], D[MIC], r2[MIC], r3[MIC], dsa[MIC], dza[MIC], ds[MIC], dz[MIC];
+  // the next factorisation's initial tiles (K's constant part, upper triangle), loaded a
+  // factorisation ahead so that the loads complete behind the solves
+  d4 Tk[NMAX / 16 * (NMAX / 16 + 1) / 2];
 };
 
 template <typename TS, int NMAX, int MIC>
@@ -216,6 +219,27 @@ struct Ctx {
   double* col;    // 128 doubles (aliases pb)
   double* vb;     // m_ineq doubles (aliases pb + 128)
 };
+
+// K's constant part as the upper-triangle tiles of factor_up (tile (J, I), J <= I, element
+// (16J + 4r + l/16, 16I + l%16) in register r), read along its rows (coalesced over the lanes).
+template <typename TS, int NMAX, int MIC>
+__device__ __forceinline__ void tile_load(const Ctx<TS, NMAX, MIC>& c, d4 (&T)[NMAX / 16 * (NMAX / 16 + 1) / 2]) {
+  constexpr int NTI = NMAX / 16;
+  const int m = c.m, l = c.l, rq = l >> 4, cq = l & 15;
+#pragma unroll
+  for (int I = 0; I < NTI; ++I) {
+#pragma unroll
+    for (int J = 0; J <= I; ++J) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * J + 4 * r + rq, cl = 16 * I + cq;
+        const bool in = row < m && cl < m;
+        const double v = c.KQ[(in ? row : 0) * m + (in ? cl : 0)];
+        T[I * (I + 1) / 2 + J][r] = in ? v : 0.0;
+      }
+    }
+  }
+}
 
 template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, const TS* Q,
@@ -347,6 +371,7 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
   }
   }
   __threadfence_block();
+  if constexpr (FAST) tile_load(c, st.Tk);
   st.qa = l < n ? (double)q[l] : ((l >= np && l < m) ? -(double)b[l - np] : 0.0);
   st.xa = 0.0;
   WSYNC();
@@ -735,21 +760,10 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     }
   }
   if (RES) c.col[l] = st.xa;
-  // tiles start as K's constant part, read along its rows (coalesced over the lanes)
+  // tiles start as K's constant part, loaded by the previous factorisation (or the problem load)
   d4 T[NTILE];
 #pragma unroll
-  for (int I = 0; I < NTI; ++I) {
-#pragma unroll
-    for (int J = 0; J <= I; ++J) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * J + 4 * r + rq, cl = 16 * I + cq;
-        const bool in = row < m && cl < m;
-        const double v = c.KQ[(in ? row : 0) * m + (in ? cl : 0)];
-        T[tidx(I, J)][r] = in ? v : 0.0;
-      }
-    }
-  }
+  for (int t = 0; t < NTILE; ++t) T[t] = st.Tk[t];
   WSYNC();
   double xc[NTI], pc[NTI], zacc[NTI];
 #pragma unroll
@@ -1239,6 +1253,7 @@ __device__ __forceinline__ void init(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) zero[cc] = 0.0;
   const double pa = -st.qa;  // -q in rows < n, b in rows np..m-1 (qa holds -b), 0 elsewhere
+  if constexpr (FAST) tile_load(c, st.Tk);  // the next factorisation's tiles, behind this solve
   solve<FAST>(c, st, pa, zero, st.h, 1.0, st.ds, st.dz);
   st.xa = st.da;
   // z = G x - h  ==  -(h - G x) = -ds from the solve above
@@ -1302,6 +1317,7 @@ __device__ __forceinline__ int pc_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX,
   const double sm = ratio * ratio * ratio * mu;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) st.r2[cc] = st.r2[cc] - (sm - st.dsa[cc] * st.dza[cc]);
+  if constexpr (FRES) tile_load(c, st.Tk);  // the next factorisation's tiles, behind this solve
   solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
   step<FRES>(c, st, false);
   return nrm < tol ? 1 : 0;
@@ -1318,6 +1334,7 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
     nrm = residuals(c, st, kappa);
     factor<false>(c, st, false, 0.0);
   }
+  if constexpr (FRES) tile_load(c, st.Tk);  // the next factorisation's tiles, behind this solve
   solve<FRES>(c, st, -st.ra, st.r2, st.r3, -1.0, st.ds, st.dz);
   step<FRES>(c, st, true);
   return nrm < tol ? 1 : 0;
