@@ -1560,23 +1560,38 @@ __global__ __launch_bounds__(256) void k_wave_bwd(Args<TS> a) {
     WSYNC();
     const double* X = c.col;
     const double* DX = c.col + 64;
-    for (int i = 0; i < n; ++i) {
-      for (int j = l; j < n; j += 64) a.dQ[pi.oQ + i * n + j] = (TS)(0.5 * (DX[i] * X[j] + X[i] * DX[j]));
-    }
-    if (l < n) a.dq[pi.oq + l] = (TS)st.da;
-    for (int cix = 0; cix < me; ++cix) {
-      const double dnu = DX[np + cix], nu = X[np + cix];
-      for (int j = l; j < n; j += 64) a.dA[pi.oA + cix * n + j] = (TS)(dnu * X[j] + nu * DX[j]);
-    }
+    // lane j owns column j (n <= 64); the per-row scalars are read eight rows at a time (uniform
+    // LDS broadcasts issued ahead of the arithmetic), so no row waits on its own LDS round trip
+    const bool cj = l < n;
+    const int jl = cj ? l : 0;
+    const double xj = X[jl], dxj = DX[jl];
+    auto rows8 = [&](int nrow, const double* r1, const double* r2, auto&& emit) {
+      int k0 = 0;
+      for (; k0 + 8 <= nrow; k0 += 8) {
+        double u1[8], u2[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          u1[u] = r1[k0 + u];
+          u2[u] = r2[k0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) emit(k0 + u, u1[u], u2[u]);
+      }
+      for (; k0 < nrow; ++k0) emit(k0, r1[k0], r2[k0]);
+    };
+    rows8(n, DX, X, [&](int i, double dxi, double xi) {
+      if (cj) a.dQ[pi.oQ + i * n + l] = (TS)(0.5 * (dxi * xj + xi * dxj));
+    });
+    if (cj) a.dq[pi.oq + l] = (TS)st.da;
+    rows8(me, DX + np, X + np, [&](int cix, double dnu, double nu) {
+      if (cj) a.dA[pi.oA + cix * n + l] = (TS)(dnu * xj + nu * dxj);
+    });
     if (l >= np && l < m) a.db[pi.ob + (l - np)] = (TS)(-st.da);
     const bool exact = a.flags & 1;
-    for (int k = 0; k < mi; ++k) {
-      const double lam = lamv[k], dlam = dlamv[k];
-      for (int j = l; j < n; j += 64) {
-        const double v = exact ? lam * dlam * X[j] + lam * DX[j] : lam * (dlam * X[j] + lam * DX[j]);
-        a.dG[pi.oG + k * n + j] = (TS)v;
-      }
-    }
+    rows8(mi, lamv, dlamv, [&](int k, double lam, double dlam) {
+      const double v = exact ? lam * dlam * xj + lam * dxj : lam * (dlam * xj + lam * dxj);
+      if (cj) a.dG[pi.oG + k * n + l] = (TS)v;
+    });
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) {
       const int k = 64 * cc + l;
