@@ -247,40 +247,51 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
                                      bool build_kq = true) {
   const int n = c.n, np = c.np, mi = c.mi, m = c.m, l = c.l, me = c.me;
   TS* Gs = const_cast<TS*>(c.Gs);
-  // G into LDS as one flat copy.  FAST: 16-byte vectors when source and destination allow it,
-  // 16 in flight per lane; otherwise 8 scalar loads in flight per lane.
+  // G into LDS as one flat copy.  FAST with 16-byte-aligned source and destination and at most
+  // 32 vectors per lane: every load is issued first (indices clamped, so no lane-divergent
+  // branch), the constant-part build below runs while they are in flight, and the stores follow
+  // it.  Otherwise: 16-byte vectors 16 in flight per lane (FAST), or 8 scalar loads in flight.
   typedef TS V __attribute__((ext_vector_type(16 / sizeof(TS))));
   constexpr int VW = 16 / sizeof(TS);
+  constexpr int GV = 32;
   const bool vec = FAST && ((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(Gs)) & 15) == 0;
-  if (vec) {
-    const int tot = mi * n, nv = tot / VW;
+  const int gtot = mi * n, gnv = gtot / VW, gnu = (gnv + 63) / 64;
+  const bool gearly = vec && gnu <= GV;
+  V gbuf[GV];
+  if (gearly) {
+    const V* Gv = reinterpret_cast<const V*>(G);
+#pragma unroll
+    for (int u = 0; u < GV; ++u) {
+      const int e = 64 * u + l;
+      if (u < gnu) gbuf[u] = Gv[e < gnv ? e : gnv - 1];
+    }
+  } else if (vec) {
     const V* Gv = reinterpret_cast<const V*>(G);
     V* Gsv = reinterpret_cast<V*>(Gs);
-    for (int e0 = 0; e0 < nv; e0 += 16 * 64) {
+    for (int e0 = 0; e0 < gnv; e0 += 16 * 64) {
       V g[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int e = e0 + 64 * u + l;
-        if (e < nv) g[u] = Gv[e];
+        if (e < gnv) g[u] = Gv[e];
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int e = e0 + 64 * u + l;
-        if (e < nv) Gsv[e] = g[u];
+        if (e < gnv) Gsv[e] = g[u];
       }
     }
-    for (int e = nv * VW + l; e < tot; e += 64) Gs[e] = G[e];
+    for (int e = gnv * VW + l; e < gtot; e += 64) Gs[e] = G[e];
   } else {
-    const int tot = mi * n;
     int e = l;
-    for (; e + 7 * 64 < tot; e += 8 * 64) {
+    for (; e + 7 * 64 < gtot; e += 8 * 64) {
       TS g[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) g[u] = G[e + 64 * u];
 #pragma unroll
       for (int u = 0; u < 8; ++u) Gs[e + 64 * u] = g[u];
     }
-    for (; e < tot; e += 64) Gs[e] = G[e];
+    for (; e < gtot; e += 64) Gs[e] = G[e];
   }
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -291,17 +302,20 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
     st.s[cc] = 1.0;
     st.z[cc] = 1.0;
   }
-  WSYNC();
-  // rows with infinite h: diag(1 * ~mask) @ G (pdip.py:176), 0 * g
+  // rows with infinite h: diag(1 * ~mask) @ G (pdip.py:176), 0 * g (after G is in LDS)
+  auto zero_inf_rows = [&]() {
+    WSYNC();
 #pragma unroll
-  for (int cc = 0; cc < MIC; ++cc) {
-    uint64_t mask = __ballot(st.hinf[cc]);
-    while (mask) {
-      const int k = 64 * cc + __builtin_ctzll(mask);
-      mask &= mask - 1;
-      for (int j = l; j < n; j += 64) Gs[k * n + j] = (TS)0 * Gs[k * n + j];
+    for (int cc = 0; cc < MIC; ++cc) {
+      uint64_t mask = __ballot(st.hinf[cc]);
+      while (mask) {
+        const int k = 64 * cc + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        for (int j = l; j < n; j += 64) Gs[k * n + j] = (TS)0 * Gs[k * n + j];
+      }
     }
-  }
+  };
+  if (!gearly) zero_inf_rows();
   // K's constant part.  FAST: every lane issues the four candidate loads of each element at
   // clamped (always valid) addresses -- Q[l][j], Q[j][l], A[j - np][l], A[l - np][j] -- for 16
   // columns at a time, then selects, so no load waits behind a lane-divergent branch.
@@ -369,6 +383,16 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
         if (j0 + u < m) c.KQ[(j0 + u) * m + l] = v[u];
     }
   }
+  }
+  if (gearly) {
+    V* Gsv = reinterpret_cast<V*>(Gs);
+#pragma unroll
+    for (int u = 0; u < GV; ++u) {
+      const int e = 64 * u + l;
+      if (u < gnu) Gsv[e < gnv ? e : gnv - 1] = gbuf[u];
+    }
+    for (int e = gnv * VW + l; e < gtot; e += 64) Gs[e] = G[e];
+    zero_inf_rows();
   }
   __threadfence_block();
   if constexpr (FAST) tile_load(c, st.Tk);
