@@ -1008,6 +1008,16 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     return v;
   };
   double dref = snap_diag();
+  // this lane's packed-factor row in each tile row I (row 16I + cq, column rq), addressed once
+  // per factorisation; a panel's stores add its first column c0 as an immediate offset
+  double* lrow[NTI];
+  bool lin[NTI];
+#pragma unroll
+  for (int I = 0; I < NTI; ++I) {
+    const int row = 16 * I + cq;
+    lrow[I] = c.Lp + opaque_v(row * (row + 1) / 2 + rq);
+    lin[I] = row < m;
+  }
 #pragma unroll
   for (int J = 0; J < NTI; ++J) {
     if (16 * J >= m) break;
@@ -1089,7 +1099,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       for (int I = 0; I < NTI; ++I) {
         if (I < J) continue;
         const int row = 16 * I + cq;
-        if (row > c0 + rq && row < m) c.Lp[row * (row + 1) / 2 + c0 + rq] = P[I] * myinv;
+        if ((I > J || row > c0 + rq) && lin[I]) lrow[I][c0] = P[I] * myinv;
       }
       // rank-4 update of every trailing tile: A = -L rows (P of the tile row, scaled), B = U rows
       // (P of the tile column); the panel's own tile row first, which the next panel reads.  The
