@@ -38,6 +38,7 @@ struct Args {
   int order_count;
   unsigned* qhead;        // work-queue head for persistent launches (null: one problem per slot)
   int lds_per_wave;       // bytes of LDS per wavefront (wavefront-per-QP kernels)
+  int64_t split_fused;    // split forward launch: problems whose relaxation runs in the solve's item
 };
 
 // Sizes and slice offsets of problem p.

@@ -1431,24 +1431,26 @@ __device__ __forceinline__ int64_t next_problem(const Args<TS>& a, int& round) {
 // Forward kernel: solve (and optionally relax) QPs, one per wavefront at a time.  FRES selects
 // factor_up with the residuals formed inside its G^T D G pass (uniform batches) over the separate
 // residual pass and factor (ragged batches).
-// Forward launch of a uniform batch with the relaxations as work items of their own: tickets
-// 0..B-1 are the PC solves, B..2B-1 the relaxations.  A solve leaves its post-processed float64
-// state (s = inf, z = 0 on rows with infinite h) after the constant matrix in its workspace slice
-// and raises a ready flag (release); the relaxation of the same problem waits for the flag
-// (acquire), reloads G into LDS and continues from that state.  A relaxation's ticket is taken
-// after its solve's, which a running wavefront holds, so the wait ends; the wait is bounded all
-// the same (after kSplitSpin polls the relaxation reports riters = -1 and stops).  The last work
-// items of the launch are then relaxations (about 4 iterations) instead of whole problems.
+// Forward launch of a uniform batch whose last problems run their relaxations as work items of
+// their own: tickets 0..F-1 solve and relax problems 0..F-1 in one item, F..B-1 are the PC
+// solves of problems F..B-1, and B..2B-F-1 their relaxations.  A split solve leaves its
+// post-processed float64 state (s = inf, z = 0 on rows with infinite h) after the constant
+// matrix in its workspace slice and raises a ready flag (release); the relaxation of the same
+// problem waits for the flag (acquire), reloads G into LDS and continues from that state.  A
+// relaxation's ticket is taken after its solve's, which a running wavefront holds, so the wait
+// ends; the wait is bounded all the same (after kSplitSpin polls the relaxation reports
+// riters = -1 and stops).  The last work items of the launch are then relaxations (about 4
+// iterations) instead of whole problems, and only the last problems pay the second load.
 constexpr int kSplitSpin = 1 << 22;
 
 template <typename TS, int NMAX, int MIC>
-__device__ __forceinline__ void wave_split(const Args<TS>& a) {
+__device__ __forceinline__ void wave_split(const Args<TS>& a, int64_t F) {
   unsigned tk = 0;
   if ((threadIdx.x & 63) == 0) tk = atomicAdd(a.qhead, 1u);
   const int64_t u = (int64_t)__builtin_amdgcn_readfirstlane(tk);
-  if (u >= 2 * a.B) return;
-  const bool rel = u >= a.B;
-  const int64_t p = rel ? u - a.B : u;
+  if (u >= 2 * a.B - F) return;
+  const bool rel = u >= a.B, fused = u < F;
+  const int64_t p = rel ? u - a.B + F : u;
   const PIdx pi = pidx(a, p);
   const auto c = make_ctx<TS, NMAX, MIC>(pi.n, pi.me, pi.mi, a.ws + pi.ows, a.lds_per_wave);
   double* sw = a.ws + pi.ows + split_state_off(pi.n, pi.me);
@@ -1479,14 +1481,21 @@ __device__ __forceinline__ void wave_split(const Args<TS>& a) {
     }
     store_state(c, st, pi, a.x, a.s, a.z, a.y, true);
     if (l == 0) { a.conv[p] = conv; a.iters[p] = it; }
-    sw[l] = st.xa;
+    if (!fused) {
+      sw[l] = st.xa;
 #pragma unroll
-    for (int cc = 0; cc < MIC; ++cc) {
-      sw[64 + 128 * cc + l] = st.hinf[cc] ? INFINITY : st.s[cc];
-      sw[128 + 128 * cc + l] = st.hinf[cc] ? 0.0 : st.z[cc];
+      for (int cc = 0; cc < MIC; ++cc) {
+        sw[64 + 128 * cc + l] = st.hinf[cc] ? INFINITY : st.s[cc];
+        sw[128 + 128 * cc + l] = st.hinf[cc] ? 0.0 : st.z[cc];
+      }
+      __threadfence();
+      if (l == 0) __hip_atomic_store(ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return;
     }
-    __threadfence();
-    if (l == 0) __hip_atomic_store(ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // relax from the post-processed point (s = inf, z = 0 on rows with infinite h)
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc)
+      if (st.hinf[cc]) { st.z[cc] = 0.0; st.s[cc] = INFINITY; }
   } else {
     st.xa = sw[l];
 #pragma unroll
@@ -1494,6 +1503,8 @@ __device__ __forceinline__ void wave_split(const Args<TS>& a) {
       st.s[cc] = sw[64 + 128 * cc + l];
       st.z[cc] = sw[128 + 128 * cc + l];
     }
+  }
+  {
     int rconv = 0, rit = 0;
     while (rit < kMaxIters && rconv == 0) {
       rconv = relax_iter<true>(c, st, a.tol, a.kappa);
@@ -1517,7 +1528,7 @@ template <typename TS, int NMAX, int MIC, bool FRES>
 __global__ __launch_bounds__(256) void k_wave_solve(Args<TS> a, int with_relax) {
   if constexpr (FRES) {
     if (with_relax == 2) {
-      wave_split<TS, NMAX, MIC>(a);
+      wave_split<TS, NMAX, MIC>(a, a.split_fused);
       return;
     }
   }

@@ -11,11 +11,16 @@ template <typename TS, int NMAX, int MIC>
 hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStream_t st, int relax) {
   if (!a.rg && !a.qhead && !a.order && relax && wpb == 1 && (int64_t)grid == a.B && a.B > 0) {
     // uniform batch: solves and relaxations as separate work items, tickets from control word 1
+    // the last 8 problems per CU (two per SIMD) run their relaxations as separate items; the rest
+    // relax in their solve's item and skip the second load
     Args<TS> b = a;
     b.qhead = a.counter + 1;
+    const int64_t split = 8 * (int64_t)qpx::host::cu_count();
+    b.split_fused = a.B > split ? a.B - split : 0;
     const int64_t flag_word = split_state_off(a.n, a.me) + kSplitStateWords;
     hipLaunchKernelGGL((k_wave_split_clear<TS>), dim3((unsigned)((a.B + 255) / 256)), dim3(256), 0, st, b, flag_word);
-    hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC, true>), dim3(2 * grid), dim3(64), a.lds_per_wave, st, b, 2);
+    hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC, true>), dim3((unsigned)(2 * a.B - b.split_fused)), dim3(64),
+                       a.lds_per_wave, st, b, 2);
     return hipGetLastError();
   }
   if (a.rg)

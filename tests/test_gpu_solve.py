@@ -69,17 +69,18 @@ def test_plain_matches_oracle(name, gen, dt, tol, xtol):
     assert rel_err(x[ok], xo[ok]).max() < xtol
 
 
-@pytest.mark.parametrize("dt", [torch.float32, torch.float64], ids=["f32", "f64"])
-def test_relaxed_graph_replays_match_eager(dt):
-    """A uniform solve + relax runs its PC solves and relaxations as separate work items (ready
-    flags and a ticket counter in the workspace, reset by a kernel at each launch): captured in a
-    hipGraph and replayed, with unrelated work in between, every replay gives the eager result
-    bit for bit, and that result is the oracle's."""
+@pytest.mark.parametrize("dt,B", [(torch.float32, 2304), (torch.float64, 256)], ids=["f32-2304", "f64-256"])
+def test_relaxed_graph_replays_match_eager(dt, B):
+    """A uniform solve + relax runs the relaxations of its last problems (the last 8 per CU) as
+    separate work items (ready flags and a ticket counter in the workspace, reset by a kernel at
+    each launch), the others in their solve's item: captured in a hipGraph and replayed, with
+    unrelated work in between, every replay gives the eager result bit for bit, and that result
+    is the oracle's.  B = 2304 mixes both kinds of item on a 256-CU part; B = 256 splits all."""
     from qpax_amd._tensors import normalize
     from qpax_amd.pdip import solve_qp_relaxed_batch
 
-    data = P.random_feasible(256, 50, 10, 100, seed=21)
-    print(f"[case] relaxed graph replay dt={dt} B=256 n=50", flush=True)
+    data = P.random_feasible(B, 50, 10, 100, seed=21)
+    print(f"[case] relaxed graph replay dt={dt} B={B} n=50", flush=True)
     bt = normalize(*[torch.tensor(np.asarray(v), dtype=dt, device="cuda") for v in data])
     eager = [t.clone() for t in solve_qp_relaxed_batch(bt, 1e-5, 1e-3)]
     torch.cuda.synchronize()
@@ -105,5 +106,5 @@ def test_relaxed_graph_replays_match_eager(dt):
     xo, rel, info = O.solve_qp_primal_fwd(*d64, solver_tol=1e-5, target_kappa=1e-3)
     ok = (info[0] == 1) & (info[2] == 1)
     assert ok.mean() > 0.9
-    xr = to_np(eager[6]).reshape(256, 50).astype(np.float64)
+    xr = to_np(eager[6]).reshape(B, 50).astype(np.float64)
     assert rel_err(xr[ok], rel[0][ok]).max() < (1e-7 if dt == torch.float64 else 1e-4)
