@@ -1,0 +1,93 @@
+"""GPU parity at the full sizes of the forward-only workloads: 10000 nonnegative least-squares QPs
+(n=5, G=-I, h=0, no equality rows) in float64 at solver_tol 1e-10 and float32 at 1e-5, and 10000
+random feasible QPs (n=20, m_eq=5, m_ineq=40) in float32 and float64 at solver_tol 1e-3, the
+default of solve_qp (qpax/pdip.py:181).
+
+Every problem of each batch is checked against the float64 oracle (flags identical, iteration
+counts within one, x within the dtype's bar), and, independently of the oracle, through properties
+that hold whatever the batch size:
+  - a converged problem's returned iterate has s > 0, z > 0 and a KKT residual below the tolerance
+    (the reference takes one more step after the residual test passes, pdip.py:87-119, which
+    shrinks it further);
+  - NNLS solutions equal scipy.optimize.nnls on the equivalent least-squares problem
+    min ||L^T x + L^-1 q|| with Q = L L^T, a library routine the nonnegative QP reduces to.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import qp_oracle as O
+from qpax_amd import problems as P
+from tests._util import rel_err, to_np
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+B_FULL = 10000
+
+CASES = [
+    # (name, generator, dtype, tol, x bar vs the oracle)
+    ("nnls-10000-f64", lambda: P.nnls(B_FULL, seed=1), np.float64, 1e-10, 1e-9),
+    ("nnls-10000-f32", lambda: P.nnls(B_FULL, seed=1), np.float32, 1e-5, 1e-4),
+    ("dense20-10000-f64", lambda: P.random_feasible(B_FULL, 20, 5, 40, seed=2), np.float64, 1e-3, 1e-9),
+    ("dense20-10000-f32", lambda: P.random_feasible(B_FULL, 20, 5, 40, seed=2), np.float32, 1e-3, 1e-4),
+]
+
+
+def _kkt_residual(data, x, s, z, y):
+    """max-norm of (Qx + q + A^T y + G^T z, s*z, Gx + s - h, Ax - b) per problem, in float64."""
+    Q, q, A, b, G, h = data
+    r1 = np.einsum("bij,bj->bi", Q, x) + q + np.einsum("bji,bj->bi", A, y) + np.einsum("bji,bj->bi", G, z)
+    r2 = s * z
+    r3 = np.einsum("bij,bj->bi", G, x) + s - h
+    r4 = np.einsum("bij,bj->bi", A, x) - b
+    parts = [np.abs(r) for r in (r1, r2, r3, r4) if r.shape[-1] > 0]
+    return np.max(np.concatenate(parts, axis=-1), axis=-1)
+
+
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+@pytest.mark.parametrize("name,gen,dt,tol,xtol", CASES, ids=[c[0] for c in CASES])
+def test_full_size_forward(name, gen, dt, tol, xtol, mode):
+    import qpax_amd
+
+    data = [np.asarray(v, dt) for v in gen()]
+    print(f"[case] {name} mode={mode} shapes={[d.shape for d in data]}", flush=True)
+    dev = [torch.as_tensor(v, device="cuda") for v in data]
+    out = qpax_amd.solve_qp(*dev, solver_tol=tol, mode=mode)
+    torch.cuda.synchronize()
+    x, s, z, y, conv, iters = [to_np(t) for t in out]
+    assert x.dtype == dt and x.shape == (B_FULL, data[1].shape[1])
+
+    d64 = [d.astype(np.float64) for d in data]
+    xo, so, zo, yo, co, io = O.solve_qp(*d64, solver_tol=tol)
+    ok = co == 1
+    assert ok.mean() > 0.99
+    np.testing.assert_array_equal(conv, co)
+    assert np.abs(iters - io).max() <= 1
+    ex = rel_err(x[ok], xo[ok])
+    print(f"{name} {mode}: x rel err max {ex.max():.2e} median {np.median(ex):.2e}; "
+          f"iters mean {iters.mean():.2f} max {iters.max()}; converged {conv.mean():.4f}")
+    assert ex.max() < xtol
+
+    # properties of the returned iterate, computed from the GPU outputs alone
+    c = conv == 1
+    x64, s64, z64, y64 = [np.asarray(v, np.float64) for v in (x, s, z, y)]
+    assert np.all(s64[c] > 0) and np.all(z64[c] > 0)
+    res = _kkt_residual([v[c] for v in d64], x64[c], s64[c], z64[c], y64[c])
+    # float32 outputs carry a rounding of about 6e-8 relative in x, s, z, y; with |Q| x of order
+    # 10-100 at these sizes that adds up to ~1e-5 to the stationarity row.
+    bar = tol if dt == np.float64 else tol + 5e-5
+    print(f"{name} {mode}: KKT residual of the returned iterate max {res.max():.2e} (bar {bar:.0e})")
+    assert res.max() < bar
+
+    if name.startswith("nnls") and dt == np.float64:
+        from scipy.optimize import nnls
+
+        Q, q = d64[0], d64[1]
+        L = np.linalg.cholesky(Q)
+        rhs = -np.linalg.solve(L, q[..., None])[..., 0]
+        idx = np.nonzero(c)[0]
+        ref = np.stack([nnls(L[i].T, rhs[i])[0] for i in idx])
+        err = np.abs(x64[idx] - ref).max()
+        print(f"{name} {mode}: |x - scipy nnls| max {err:.2e}")
+        assert err < 1e-7
