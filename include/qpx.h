@@ -157,7 +157,13 @@ QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batc
  * offsets; outputs reuse them (x at q's offset, s and z at h's, y at b's, dQ at Q's, ...).
  * A plan is bound to the device current at creation and holds device tables for the size pattern;
  * calls through it allocate nothing and do not synchronise (capturable in a hipGraph).  Calls that
- * share one workspace must be stream-ordered. */
+ * share one workspace must be stream-ordered.
+ * A call runs its buckets on up to four lanes: the caller's stream and up to three side streams the
+ * plan owns, forked from the caller's stream and joined back into it, so the call is still ordered
+ * on the caller's stream as one unit (a captured call is a graph with one branch per lane).  The
+ * environment variable QPX_RAGGED_LANES, read at plan creation, caps the lanes (1: every bucket on
+ * the caller's stream).  One plan serves one call at a time: concurrent calls on one plan from
+ * several host threads race on its fork/join events. */
 typedef struct qpx_plan qpx_plan;
 
 /* Buckets the problems by kernel variant, orders each bucket longest-first and uploads the tables

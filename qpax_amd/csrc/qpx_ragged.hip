@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -74,6 +75,13 @@ struct qpx_plan {
   std::vector<int> order;  // host copy of the device order
   RaggedEntry* d_table = nullptr;
   int* d_order = nullptr;
+  // side streams (default priority, non-blocking) that run buckets beside the caller's stream, and
+  // the events that fork them from it and join them back into it; created with the plan
+  static constexpr int kSide = 3;
+  int n_side = 0;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSide] = {};
 };
 
 namespace {
@@ -178,29 +186,63 @@ hipError_t reset_queues(const qpx_plan* p, void* ws, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Launches every bucket, bucket i on lane i % lanes: lane 0 is the caller's stream, the others the
+// plan's side streams, forked from the caller's stream after the queue reset and joined back into it
+// at the end.  Buckets share no memory (own queue head, own problems' outputs and workspace slices),
+// so their order does not change any result.  Run side by side, a persistent workgroup that finds
+// its bucket's queue empty frees its CU for the workgroups of a bucket on another lane, instead of
+// each bucket's tail leaving CUs idle before the next launch starts.  Event records and waits are
+// legal under stream capture: a captured call becomes a graph with one branch per lane.
+template <typename L>
+qpx_status fan_out(const qpx_plan* p, hipStream_t st, L&& launch) {
+  const int nb = (int)p->buckets.size();
+  const int lanes = std::min(nb, 1 + p->n_side);
+  if (lanes <= 1) {
+    for (int i = 0; i < nb; ++i) QPX_HIP(launch(i, st));
+    return QPX_OK;
+  }
+  QPX_HIP(hipEventRecord(p->fork, st));
+  for (int k = 1; k < lanes; ++k) QPX_HIP(hipStreamWaitEvent(p->side[k - 1], p->fork, 0));
+  qpx_status st_launch = QPX_OK;
+  for (int i = 0; i < nb && st_launch == QPX_OK; ++i) {
+    const int k = i % lanes;
+    if (launch(i, k == 0 ? st : p->side[k - 1]) != hipSuccess)
+      st_launch = fail(QPX_ERR_HIP, "launch of bucket %d failed: %s", i, hipGetErrorString(hipGetLastError()));
+  }
+  // join every forked lane, also after a failed launch, so that a capture is never left open on a
+  // side stream
+  for (int k = 1; k < lanes; ++k) {
+    QPX_HIP(hipEventRecord(p->join[k - 1], p->side[k - 1]));
+    QPX_HIP(hipStreamWaitEvent(st, p->join[k - 1], 0));
+  }
+  return st_launch;
+}
+
 template <typename TS>
 qpx_status run_fwd(const qpx_plan* p, const FwdIO& io, bool relax, void* ws, hipStream_t st, int only = -1) {
   QPX_HIP(reset_queues(p, ws, st));
-  for (int i = 0; i < (int)p->buckets.size(); ++i) {
-    if (only >= 0 && i != only) continue;
+  auto launch = [&](int i, hipStream_t s) {
     const Bucket& b = p->buckets[i];
     auto a = bucket_args<TS>(p, b, i, ws);
     fill_fwd(a, io, relax);
-    QPX_HIP(launch_bucket_fwd<TS>(a, b, st, relax ? 1 : 0));
+    return launch_bucket_fwd<TS>(a, b, s, relax ? 1 : 0);
+  };
+  if (only >= 0) {
+    QPX_HIP(launch(only, st));
+    return QPX_OK;
   }
-  return QPX_OK;
+  return fan_out(p, st, launch);
 }
 
 template <typename TS>
 qpx_status run_bwd(const qpx_plan* p, const BwdIO& io, void* ws, hipStream_t st) {
   QPX_HIP(reset_queues(p, ws, st));
-  for (int i = 0; i < (int)p->buckets.size(); ++i) {
+  return fan_out(p, st, [&](int i, hipStream_t s) {
     const Bucket& b = p->buckets[i];
     auto a = bucket_args<TS>(p, b, i, ws);
     fill_bwd(a, io);
-    QPX_HIP(launch_bucket_bwd<TS>(a, b, st));
-  }
-  return QPX_OK;
+    return launch_bucket_bwd<TS>(a, b, s);
+  });
 }
 
 qpx_status check_call(const qpx_plan* p, size_t ws_bytes, void* ws) {
@@ -347,6 +389,24 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
       return fail(QPX_ERR_HIP, "allocating or uploading the plan's device tables failed");
     }
   }
+  // lanes for the buckets: QPX_RAGGED_LANES (1 = every bucket on the caller's stream), default
+  // 1 + kSide, never more than there are buckets
+  int lanes = 1 + qpx_plan::kSide;
+  if (const char* env = std::getenv("QPX_RAGGED_LANES")) lanes = std::atoi(env);
+  lanes = std::max(1, std::min({lanes, 1 + qpx_plan::kSide, (int)p->buckets.size()}));
+  bool ok = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; ok && k < lanes - 1; ++k) {
+    ok = hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking) == hipSuccess;
+    if (ok && hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming) != hipSuccess) {
+      (void)hipStreamDestroy(p->side[k]);
+      ok = false;
+    }
+    if (ok) p->n_side = k + 1;
+  }
+  if (!ok) {
+    qpx_plan_destroy(p);
+    return fail(QPX_ERR_HIP, "creating the plan's side streams or events failed");
+  }
   *out = p;
   return QPX_OK;
 }
@@ -358,6 +418,11 @@ QPX_API void qpx_plan_destroy(qpx_plan* p) {
   if (cur != p->device) (void)hipSetDevice(p->device);
   if (p->d_table) (void)hipFree(p->d_table);
   if (p->d_order) (void)hipFree(p->d_order);
+  for (int k = 0; k < p->n_side; ++k) {
+    (void)hipEventDestroy(p->join[k]);
+    (void)hipStreamDestroy(p->side[k]);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   if (cur >= 0 && cur != p->device) (void)hipSetDevice(cur);
   delete p;
 }
