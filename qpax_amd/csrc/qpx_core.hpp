@@ -223,6 +223,17 @@ __host__ __device__ inline int gtdg_lds_bytes(int n) {
   const int a = kGtdgRows * gtdg_stride(n) + kGtdgRows, b = kPanelLd * 16 * ((n + 15) / 16);
   return 8 * (a > b ? a : b);
 }
+// W = L^{-1} A^T (n x me float64) is formed in LDS, aliasing the G^T D G chunk and the Cholesky
+// panel (both finished by then), when the factorisation is the tiled one and W fits this budget.
+constexpr int kWLdsMax = 128 * 1024;
+__host__ __device__ inline bool w_in_lds(int n, int me) {
+  return n >= kMfmaMinN && n <= kMfmaMaxN && me > 0 && 8 * n * me <= kWLdsMax;
+}
+// Dynamic LDS bytes of the generic kernels for a problem of n variables and me equality rows.
+__host__ __device__ inline int generic_lds_bytes(int n, int me) {
+  const int g = gtdg_lds_bytes(n), w = w_in_lds(n, me) ? 8 * n * me : 0;
+  return g > w ? g : w;
+}
 
 // Sum over each group of 16 lanes (DPP row butterflies); every lane of the group receives it.
 template <int CTRL>
@@ -535,6 +546,90 @@ __device__ void block_w_tiled(const double* L, const double* rcp, int n, int me,
   }
 }
 
+// W = L^{-1} A^T with W (n x me, row-major) in LDS, L from block_cholesky_tiled.  Wavefront 0
+// eliminates each 16-row block (one lane per column of W, as block_w_tiled); the rows below then
+// receive -L[rows, block] W[block, :] as 16x16 tiles, four v_mfma_f64_16x16x4f64 each, spread over
+// the wavefronts: one batch of L loads per tile instead of one dependent global round trip per row.
+template <int NT>
+__device__ void block_w_lds(const double* L, const double* rcp, int n, int me, double* Wl) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const int nt = (n + 15) / 16, ct = (me + 15) / 16;
+  for (int k = 0; k < nt; ++k) {
+    const int c0 = 16 * k, cw = n - c0 < 16 ? n - c0 : 16;
+    if (w == 0) {
+      for (int cg = 0; cg < me; cg += 64) {
+        const int c = cg + l;
+        double wj[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          double wi = (i < cw && c < me) ? Wl[(c0 + i) * me + c] : 0.0;
+#pragma unroll
+          for (int j = 0; j < i; ++j) wi -= (i < cw ? L[(c0 + i) * n + c0 + j] : 0.0) * wj[j];
+          wi *= i < cw ? rcp[c0 + i] : 0.0;
+          wj[i] = wi;
+          if (i < cw && c < me) Wl[(c0 + i) * me + c] = wi;
+        }
+      }
+    }
+    __syncthreads();
+    const int T = (nt - k - 1) * ct;
+    for (int idx = w; idx < T; idx += W) {
+      const int R = k + 1 + idx / ct, C = idx - (idx / ct) * ct;
+      const int ia = 16 * R + cq, jb = 16 * C + cq;
+      double av[4], bv[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int kk = 4 * s + rq;  // column of the block
+        av[s] = (ia < n && kk < cw) ? -L[ia * n + c0 + kk] : 0.0;
+        bv[s] = (kk < cw && jb < me) ? Wl[(c0 + kk) * me + jb] : 0.0;
+      }
+      d4v acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * R + 4 * r + rq;
+        acc[r] = (i < n && jb < me) ? Wl[i * me + jb] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * R + 4 * r + rq;
+        if (i < n && jb < me) Wl[i * me + jb] = acc[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Lower triangle of F = W^T W (me x me, row-major, into global F) from W in LDS: 16x16 tiles
+// (C >= D) accumulated over the n rows of W with v_mfma_f64_16x16x4f64, one tile per wavefront
+// at a time.
+template <int NT>
+__device__ void wtw_mfma(const double* Wl, int n, int me, double* F) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const int ct = (me + 15) / 16, T = ct * (ct + 1) / 2;
+  for (int idx = w; idx < T; idx += W) {
+    int C = 0;
+    while ((C + 1) * (C + 2) / 2 <= idx) ++C;
+    const int D = idx - C * (C + 1) / 2;
+    const int ca = 16 * C + cq, cb = 16 * D + cq;
+    d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += 4) {
+      const int k = k0 + rq;
+      const double av = (k < n && ca < me) ? Wl[k * me + ca] : 0.0;
+      const double bv = (k < n && cb < me) ? Wl[k * me + cb] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * C + 4 * r + rq;
+      if (i < me && cb <= i) F[i * me + cb] = acc[r];
+    }
+  }
+}
+
 // Triangular solves with a factor from block_cholesky[_tiled]: blocked from m = 64 on.
 template <int NT>
 __device__ void block_fsub(const double* L, const double* rcp, int m, double* b) {
@@ -575,6 +670,24 @@ __device__ void stage_factor(const View& v, int dmode) {
     block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
   }
   if (me == 0) return;
+  if (w_in_lds(n, me)) {
+    // W in LDS (aliasing the finished G^T D G chunk and Cholesky panel), its trailing updates and
+    // F = W^T W on the matrix cores; W is copied to the workspace for the solves
+    extern __shared__ __align__(16) unsigned char qpx_smem[];
+    double* Wl = reinterpret_cast<double*>(qpx_smem);
+    __syncthreads();
+    for (int e = tid; e < n * me; e += NT) {
+      const int i = e / me, c = e - i * me;
+      Wl[e] = v.A[c * n + i];
+    }
+    __syncthreads();
+    block_w_lds<NT>(v.H, v.hrcp, n, me, Wl);
+    wtw_mfma<NT>(Wl, n, me, v.F);
+    for (int e = tid; e < n * me; e += NT) v.W[e] = Wl[e];
+    __syncthreads();
+    block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
+    return;
+  }
   // W = L^{-1} A^T, all me columns at once: column-oriented forward substitution.
   for (int e = tid; e < n * me; e += NT) {
     const int i = e / me, c = e - i * me;

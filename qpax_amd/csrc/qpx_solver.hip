@@ -573,7 +573,7 @@ bool wave_plan(qpx::Args<TS>& a, qpx::host::WaveVariant* v) {
 
 template <int NT, typename TS>
 qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
-  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -621,7 +621,7 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
 
 template <int NT, typename TS>
 qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
-  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -688,7 +688,7 @@ qpx_status backward_impl(int mode, int64_t B, int n, int me, int mi, const void*
 // ------------------------------------ elastic -----------------------------------------------
 template <int NT, typename TS>
 qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
-  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -725,7 +725,7 @@ qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st)
 
 template <int NT, typename TS>
 qpx_status run_e_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
-  const int lds = qpx::gtdg_lds_bytes(a.n);  // matrix-core G^T D G staging (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
