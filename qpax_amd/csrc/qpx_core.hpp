@@ -440,95 +440,53 @@ __device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rc
 
 // Blocked forward substitution L y = b (L from block_cholesky[_tiled]; b overwritten): wavefront 0
 // eliminates inside each 16-row block with lane broadcasts, then all threads apply the block to
-// the rows below it.  L is read-only here, so every thread issues the loads of its first row below
-// the block (and wavefront 0 those of the block and its reciprocals) before the elimination: one
-// exposed global round trip per block.  Rows below a block exist only when the block is full.
+// the rows below it.
 template <int NT>
 __device__ __forceinline__ void fsub_blocks(const double* L, const double* rcp, int m, double* b) {
   const int tid = threadIdx.x, l = tid & 63;
   const int nt = (m + 15) / 16;
   for (int k = 0; k < nt; ++k) {
     const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
-    const int r0 = c0 + 16 + tid;
-    double lr[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) lr[j] = r0 < m ? L[r0 * m + c0 + j] : 0.0;
     if (tid < 64) {
-      double lv[16], rv[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        lv[j] = (l > j && l < cw) ? L[(c0 + l) * m + c0 + j] : 0.0;
-        rv[j] = j < cw ? rcp[c0 + j] : 0.0;
-      }
       double bi = l < cw ? b[c0 + l] : 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j < cw) {
-          const double yj = __shfl(bi, j) * rv[j];
-          bi = (l == j) ? yj : bi - lv[j] * yj;
-        }
+      for (int j = 0; j < cw; ++j) {
+        const double yj = __shfl(bi, j) * rcp[c0 + j];
+        const double lv = (l > j && l < cw) ? L[(c0 + l) * m + c0 + j] : 0.0;
+        bi = (l == j) ? yj : bi - lv * yj;
       }
       if (l < cw) b[c0 + l] = bi;
     }
     __syncthreads();
-    if (r0 < m) {
+    for (int r = c0 + 16 + tid; r < m; r += NT) {
+      const double* lr = L + r * m + c0;
       double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; j += 2) { a0 += lr[j] * b[c0 + j]; a1 += lr[j + 1] * b[c0 + j + 1]; }
-      b[r0] -= a0 + a1;
-    }
-    for (int r = r0 + NT; r < m; r += NT) {
-      const double* lrow = L + r * m + c0;
-      double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; j += 2) { a0 += lrow[j] * b[c0 + j]; a1 += lrow[j + 1] * b[c0 + j + 1]; }
+      int j = 0;
+      for (; j + 2 <= cw; j += 2) { a0 += lr[j] * b[c0 + j]; a1 += lr[j + 1] * b[c0 + j + 1]; }
+      if (j < cw) a0 += lr[j] * b[c0 + j];
       b[r] -= a0 + a1;
     }
     __syncthreads();
   }
 }
 
-// Blocked back substitution L^T x = b (b overwritten), blocks from the last; loads issued as in
-// fsub_blocks (the first row above the block per thread, the block for wavefront 0).
+// Blocked back substitution L^T x = b (b overwritten), blocks from the last.
 template <int NT>
 __device__ __forceinline__ void bsub_t_blocks(const double* L, const double* rcp, int m, double* b) {
   const int tid = threadIdx.x, l = tid & 63;
   const int nt = (m + 15) / 16;
   for (int k = nt - 1; k >= 0; --k) {
     const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
-    const int r0 = tid;
-    double lc[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) lc[j] = (r0 < c0 && j < cw) ? L[(c0 + j) * m + r0] : 0.0;
     if (tid < 64) {
-      double lv[16], rv[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        lv[j] = (l < j && j < cw) ? L[(c0 + j) * m + c0 + l] : 0.0;
-        rv[j] = j < cw ? rcp[c0 + j] : 0.0;
-      }
       double bi = l < cw ? b[c0 + l] : 0.0;
-#pragma unroll
-      for (int j = 15; j >= 0; --j) {
-        if (j < cw) {
-          const double xj = __shfl(bi, j) * rv[j];
-          bi = (l == j) ? xj : bi - lv[j] * xj;
-        }
+      for (int j = cw - 1; j >= 0; --j) {
+        const double xj = __shfl(bi, j) * rcp[c0 + j];
+        const double lv = (l < j) ? L[(c0 + j) * m + c0 + l] : 0.0;
+        bi = (l == j) ? xj : bi - lv * xj;
       }
       if (l < cw) b[c0 + l] = bi;
     }
     __syncthreads();
-    if (r0 < c0) {
-      double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; j += 2) {
-        // lc[j] = 0 for j >= cw; the clamped index keeps the b read inside the vector
-        a0 += lc[j] * b[c0 + (j < cw ? j : 0)];
-        a1 += lc[j + 1] * b[c0 + (j + 1 < cw ? j + 1 : 0)];
-      }
-      b[r0] -= a0 + a1;
-    }
-    for (int r = r0 + NT; r < c0; r += NT) {
+    for (int r = tid; r < c0; r += NT) {
       double a0 = 0.0, a1 = 0.0;
       int j = 0;
       for (; j + 2 <= cw; j += 2) {
@@ -742,7 +700,9 @@ __device__ void stage_factor(const View& v, int dmode) {
     block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
   }
   if (me == 0) return;
-  if (w_in_lds(n, me)) {
+  // (256-thread kernels only: in the 64/128-thread ones, which also serve n < 64, the LDS path's
+  // registers would lower the occupancy of every launch)
+  if (NT >= 256 && w_in_lds(n, me)) {
     // W in LDS (aliasing the finished G^T D G chunk and Cholesky panel), its trailing updates and
     // F = W^T W on the matrix cores; W is copied to the workspace for the solves
     extern __shared__ __align__(16) unsigned char qpx_smem[];
