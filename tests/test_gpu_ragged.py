@@ -170,6 +170,32 @@ def test_graph_capture_replays_match_eager():
         (torch.randn(1024, device="cuda") * 2).sum().item()
 
 
+def test_lanes_match_serial_bitwise(monkeypatch):
+    """A plan runs its buckets on concurrent lanes (the caller's stream plus side streams, forked
+    and joined with events); QPX_RAGGED_LANES=1 at plan creation keeps them all on the caller's
+    stream.  Buckets share no memory, so both give the same bits, forward and backward, with all
+    three kernel kinds in the batch."""
+    probs = P.uneven(40, seed=45, n_lo=8, n_hi=128)
+    for sd in (3, 4):
+        Q, q, A, b, G, h = P.random_feasible(1, 150, 20, 200, seed=sd)
+        probs[5 * sd] = (Q[0], q[0], A[0], b[0], G[0], h[0])
+    print("[case] ragged lanes vs serial f64 B=40", flush=True)
+    res = []
+    for lanes in ("1", None):
+        if lanes is None:
+            monkeypatch.delenv("QPX_RAGGED_LANES", raising=False)
+        else:
+            monkeypatch.setenv("QPX_RAGGED_LANES", lanes)
+        plan, ops = _plan(probs, torch.float64)
+        assert {bk["kind"] for bk in plan.buckets()} == {"wave", "group", "block"}
+        out = [t.clone() for t in plan.solve_relaxed(ops, 1e-5, 1e-3)]
+        grads = [t.clone() for t in plan.diff(ops, out[6:10], 2.0 * (out[0] - 1.0))]
+        torch.cuda.synchronize()
+        res.append(out + grads)
+    for a, b in zip(*res):
+        assert torch.equal(a, b)
+
+
 def test_uneven_full_size_f32():
     """16384 mixed problems (n ~ U{8..128}, m_ineq ~ U{n..3n}, m_eq ~ U{0..n/4}, seed 4), float32
     forward + backward in one tuned plan; a sample checked per problem against the float64 oracle."""
