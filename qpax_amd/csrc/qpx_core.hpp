@@ -252,27 +252,29 @@ __device__ __forceinline__ double sum16(double v) {
 
 // out[k] = sum_j M[k n + j] x[j] for every row k < rows: 16 lanes per row read it coalesced and
 // meet in a DPP reduction (the large-size form of the row products, n >= kMfmaMinN).
-template <int NT>
-__device__ void rows_dot16(const double* M, int rows, int n, const double* x, double* out) {
+template <int NT, typename MT>
+__device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const double* x, double* out) {
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, rq = l >> 4, cq = l & 15;
   for (int kb = 4 * w; kb < rows; kb += 4 * W) {
     const int k = kb + rq;
-    const double* mr = M + (int64_t)(k < rows ? k : rows - 1) * n;
+    const MT* mr = M + (int64_t)(k < rows ? k : rows - 1) * n;
     double a0 = 0.0, a1 = 0.0;
     int j = cq;
     for (; j + 16 < n; j += 32) {
-      a0 += mr[j] * x[j];
-      a1 += mr[j + 16] * x[j + 16];
+      a0 += (double)mr[j] * x[j];
+      a1 += (double)mr[j + 16] * x[j + 16];
     }
-    if (j < n) a0 += mr[j] * x[j];
+    if (j < n) a0 += (double)mr[j] * x[j];
     const double v = sum16(a0 + a1);
     if (cq == 0 && k < rows) out[k] = v;
   }
 }
 
-template <int NT>
-__device__ void gtdg_mfma(const View& v, double* chunk) {
+// G is read from `G` (the workspace copy, or for the elastic solve the caller's array, whose
+// element type GT widens exactly to the copy's float64 values).
+template <int NT, typename GT>
+__device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT* G) {
   constexpr int W = NT / 64;
   // 256-thread blocks prefetch the next chunk into registers under the MFMAs and keep 16
   // accumulators per wavefront; smaller blocks (n <= 96) load in place and keep 8
@@ -299,7 +301,7 @@ __device__ void gtdg_mfma(const View& v, double* chunk) {
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
       const int64_t g = base + tid + NT * u;
-      reg[u] = (dst[u] >= 0 && g < total) ? v.G[g] : 0.0;
+      reg[u] = (dst[u] >= 0 && g < total) ? (double)G[g] : 0.0;
     }
     dreg = (tid < kGtdgRows && k0 + tid < mi) ? v.D[k0 + tid] : 0.0;
   };
@@ -336,7 +338,7 @@ __device__ void gtdg_mfma(const View& v, double* chunk) {
       } else {
         for (int r = 0; r < kGtdgRows; ++r) {
           const int k = k0 + r;
-          for (int c = tid; c < n; c += NT) chunk[r * ldl + c] = k < mi ? v.G[(int64_t)k * n + c] : 0.0;
+          for (int c = tid; c < n; c += NT) chunk[r * ldl + c] = k < mi ? (double)G[(int64_t)k * n + c] : 0.0;
         }
         for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? v.D[k0 + r] : 0.0;
       }
@@ -534,7 +536,7 @@ __device__ void block_bsub_t_tiled(const double* L, const double* rcp, int m, do
 // wavefront 0 eliminates each 16-row block (one lane per column of W), then every wavefront
 // applies the block to its share of the rows below.
 template <int NT>
-__device__ void block_w_tiled(const double* L, const double* rcp, int n, int me, double* Wm) {
+__device__ __forceinline__ void block_w_tiled(const double* L, const double* rcp, int n, int me, double* Wm) {
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int nt = (n + 15) / 16;
@@ -673,21 +675,21 @@ __device__ void block_bsub_t(const double* L, const double* rcp, int m, double* 
   else block_bsub_t_scalar<NT>(L, rcp, m, b);
 }
 
-template <int NT>
-__device__ void stage_factor(const View& v, int dmode) {
+template <int NT, typename GT>
+__device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const GT* G) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   if (dmode != kDgiven)
     for (int k = tid; k < mi; k += NT) v.D[k] = dmode == kDunit ? 1.0 : v.z[k] / v.s[k];
   __syncthreads();
   if (n >= kMfmaMinN && n <= kMfmaMaxN) {
     extern __shared__ __align__(16) unsigned char qpx_smem[];
-    gtdg_mfma<NT>(v, reinterpret_cast<double*>(qpx_smem));
+    gtdg_mfma<NT>(v, reinterpret_cast<double*>(qpx_smem), G);
   } else {
     for (int e = tid; e < n * n; e += NT) {
       const int i = e / n, j = e - i * n;
       if (j > i) continue;
       double acc = 0.0;
-      for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * (v.D[k] * v.G[k * n + j]);
+      for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * (v.D[k] * (double)G[k * n + j]);
       v.H[e] = v.Q[e] + acc;
     }
   }
@@ -747,6 +749,11 @@ __device__ void stage_factor(const View& v, int dmode) {
   }
   __syncthreads();
   block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_factor(const View& v, int dmode) {
+  stage_factor_g<NT, double>(v, dmode, v.G);
 }
 
 // Core of the reduced solve: given p1 (in v.u, length n) and v4, produce dy (v.dy) and dx.

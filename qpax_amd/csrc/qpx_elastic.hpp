@@ -107,13 +107,14 @@ __device__ void e_init(EView& e, double* red) {
 // Residuals (qpax/elastic_qp.py:105-110; relaxed: :256-261).  The reference's relaxed residuals
 // flip the sign of t and of r2 (sign_t = +1, sign_r2 = -1); the forward uses sign_t = -1,
 // sign_r2 = +1.  Returns the entry convergence flag.
-template <int NT>
-__device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2, double tol, double* red) {
+template <int NT, typename GT>
+__device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2, double tol, double* red,
+                           const GT* G) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   const bool wide = n >= kMfmaMinN;
   if (wide) {  // G x into r6 and A x into r4 first; Q x by columns (Q is symmetric), coalesced
-    rows_dot16<NT>(v.G, mi, n, v.x, e.r6);
+    rows_dot16<NT>(G, mi, n, v.x, e.r6);
     rows_dot16<NT>(v.A, me, n, v.x, v.r4);
     __syncthreads();
   }
@@ -124,7 +125,7 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
       for (int j = 0; j < n; ++j) acc += v.Q[j * n + i] * v.x[j];
     else
       for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
-    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * e.z2[k];
+    for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * e.z2[k];
     for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
     v.r1[i] = acc;
     m = nmax(m, fabs(acc));
@@ -133,7 +134,7 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
     double g = 0.0;
     if (wide) g = e.r6[k];
     else
-      for (int j = 0; j < n; ++j) g += v.G[k * n + j] * v.x[j];
+      for (int j = 0; j < n; ++j) g += (double)G[k * n + j] * v.x[j];
     const double r2 = sign_r2 * (-e.z1[k] - e.z2[k] + e.pen);
     const double r3 = e.s1[k] * e.z1[k] - kappa;
     const double r4 = e.s2[k] * e.z2[k] - kappa;
@@ -156,19 +157,19 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
 }
 
 // Factorisation for Algorithm 5 at (s1, z1, s2, z2): D = 1/a3.
-template <int NT>
-__device__ void e_factor(EView& e) {
+template <int NT, typename GT>
+__device__ void e_factor(EView& e, const GT* G) {
   const View& v = e.v;
   for (int k = threadIdx.x; k < v.mi; k += NT) v.D[k] = 1.0 / (e.s1[k] / e.z1[k] + e.s2[k] / e.z2[k]);
-  stage_factor<NT>(v, kDgiven);
+  stage_factor_g<NT, GT>(v, kDgiven, G);
 }
 
 // Algorithm 5 with right-hand side sg * (R1, R2, R3, R4, R5, R6, R7) (R7: the equality rows) using
 // the current factorisation.  Writes dx, dy (in v) and dt, ds1, ds2, dz1, dz2 (given arrays).
-template <int NT>
+template <int NT, typename GT>
 __device__ void e_solve(EView& e, const double* R1, const double* R2, const double* R3, const double* R4,
                         const double* R5, const double* R6, const double* R7, double sg, double* dt,
-                        double* ds1, double* ds2, double* dz1, double* dz2) {
+                        double* ds1, double* ds2, double* dz1, double* dz2, const GT* G) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   for (int k = tid; k < mi; k += NT) {
@@ -181,7 +182,7 @@ __device__ void e_solve(EView& e, const double* R1, const double* R2, const doub
   __syncthreads();
   for (int i = tid; i < n; i += NT) {
     double acc = 0.0;
-    for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * e.pa3[k];
+    for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * e.pa3[k];
     v.u[i] = sg * R1[i] - acc;
   }
   for (int c = tid; c < me; c += NT) v.dy[c] = sg * R7[c];
@@ -189,14 +190,14 @@ __device__ void e_solve(EView& e, const double* R1, const double* R2, const doub
   reduced_solve<NT>(v, v.dy, v.dx);
   const bool wide = n >= kMfmaMinN;
   if (wide) {  // G dx into dz2 first
-    rows_dot16<NT>(v.G, mi, n, v.dx, dz2);
+    rows_dot16<NT>(G, mi, n, v.dx, dz2);
     __syncthreads();
   }
   for (int k = tid; k < mi; k += NT) {
     double g = 0.0;
     if (wide) g = dz2[k];
     else
-      for (int j = 0; j < n; ++j) g += v.G[k * n + j] * v.dx[j];
+      for (int j = 0; j < n; ++j) g += (double)G[k * n + j] * v.dx[j];
     const double a3 = e.s1[k] / e.z1[k] + e.s2[k] / e.z2[k];
     const double d2 = (e.p1[k] + g) / a3;
     const double d1 = -(sg * R2[k]) - d2;
@@ -266,25 +267,25 @@ __device__ void e_step(EView& e, double* red) {
 }
 
 // One predictor-corrector iteration (qpax/elastic_qp.py:99-174); returns the entry flag.
-template <int NT>
-__device__ int e_pc_iteration(EView& e, double tol, double* red) {
-  const int conv = e_residuals<NT>(e, 0.0, -1.0, 1.0, tol, red);
-  e_factor<NT>(e);
-  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a);
+template <int NT, typename GT>
+__device__ int e_pc_iteration(EView& e, double tol, double* red, const GT* G) {
+  const int conv = e_residuals<NT>(e, 0.0, -1.0, 1.0, tol, red, G);
+  e_factor<NT>(e, G);
+  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a, G);
   e_center<NT>(e, red);
-  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2);
+  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
   e_step<NT>(e, red);
   return conv;
 }
 
 // One relaxed Newton iteration (qpax/elastic_qp.py:235-316); consistent != 0 uses the forward's
 // signs of t and r2.
-template <int NT>
-__device__ int e_relax_iteration(EView& e, double tol, double kappa, int consistent, double* red) {
-  const int conv = consistent ? e_residuals<NT>(e, kappa, -1.0, 1.0, tol, red)
-                              : e_residuals<NT>(e, kappa, 1.0, -1.0, tol, red);
-  e_factor<NT>(e);
-  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2);
+template <int NT, typename GT>
+__device__ int e_relax_iteration(EView& e, double tol, double kappa, int consistent, double* red, const GT* G) {
+  const int conv = consistent ? e_residuals<NT>(e, kappa, -1.0, 1.0, tol, red, G)
+                              : e_residuals<NT>(e, kappa, 1.0, -1.0, tol, red, G);
+  e_factor<NT>(e, G);
+  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
   e_step<NT>(e, red);
   return conv;
 }
@@ -338,8 +339,8 @@ __global__ __launch_bounds__(NT) void k_e_resid(Args<TS> a, int relax) {
   if (threadIdx.x == 0) v.ctl[kActive] = act ? 1 : 0;
   if (!act) return;
   const bool fwd_signs = !relax || (a.flags & 2);
-  const int conv = fwd_signs ? e_residuals<NT>(e, relax ? a.kappa : 0.0, -1.0, 1.0, a.tol, red)
-                             : e_residuals<NT>(e, a.kappa, 1.0, -1.0, a.tol, red);
+  const int conv = fwd_signs ? e_residuals<NT>(e, relax ? a.kappa : 0.0, -1.0, 1.0, a.tol, red, e.v.G)
+                             : e_residuals<NT>(e, a.kappa, 1.0, -1.0, a.tol, red, e.v.G);
   if (threadIdx.x == 0) v.ctl[5] = conv;
 }
 
@@ -347,15 +348,15 @@ template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_e_factor(Args<TS> a) {
   EView e = eview_of(a, blockIdx.x);
   if (!e.v.ctl[kActive]) return;
-  e_factor<NT>(e);
+  e_factor<NT>(e, e.v.G);
 }
 
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_e_solve(Args<TS> a, int affine) {
   EView e = eview_of(a, blockIdx.x);
   if (!e.v.ctl[kActive]) return;
-  if (affine) e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a);
-  else e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2);
+  if (affine) e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a, e.v.G);
+  else e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, e.v.G);
 }
 
 template <int NT, typename TS>
@@ -420,9 +421,12 @@ __global__ __launch_bounds__(NT) void k_e_fused_solve(Args<TS> a, int with_relax
   EView e = eview_of(a, p);
   QPX_E_LOAD(NT, TS, e, a, p);
   e_init<NT>(e, red);
+  // G straight from the caller's array (the workspace copy holds the same values widened to
+  // float64): half the bytes of the per-iteration G passes for float32 inputs
+  const TS* Gin = a.G + p * a.mi * a.n;
   int conv = 0, it = 0;
   while (it < kMaxIters && conv == 0) {
-    conv = e_pc_iteration<NT>(e, a.tol, red);
+    conv = e_pc_iteration<NT>(e, a.tol, red, Gin);
     ++it;
   }
   if (threadIdx.x == 0) { e.v.ctl[kConv] = conv; e.v.ctl[kIter] = it; }
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_e_fused_solve(Args<TS> a, int with_relax
   if (!with_relax) return;
   int rconv = 0, rit = 0;
   while (rit < kMaxIters && rconv == 0) {
-    rconv = e_relax_iteration<NT>(e, a.tol, a.kappa, a.flags & 2, red);
+    rconv = e_relax_iteration<NT>(e, a.tol, a.kappa, a.flags & 2, red, Gin);
     ++rit;
   }
   if (threadIdx.x == 0) { e.v.ctl[kRelaxConv] = rconv; e.v.ctl[kRelaxIter] = rit; }
@@ -463,7 +467,7 @@ template <int NT, typename TS>
 __device__ void e_bwd_solve_write(const Args<TS>& a, EView& e, int64_t p) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
-  e_solve<NT>(e, v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2);
+  e_solve<NT>(e, v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, v.G);
   const bool exact = a.flags & 1;
   for (int i = tid; i < n * n; i += NT) {
     const int r = i / n, c = i - r * n;
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(NT) void k_e_bwd_load(Args<TS> a) {
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_e_bwd_factor(Args<TS> a) {
   EView e = eview_of(a, blockIdx.x);
-  e_factor<NT>(e);
+  e_factor<NT>(e, e.v.G);
 }
 
 template <int NT, typename TS>
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_e_fused_bwd(Args<TS> a) {
   const int64_t p = blockIdx.x;
   EView e = eview_of(a, p);
   e_bwd_load<NT, TS>(a, e, p);
-  e_factor<NT>(e);
+  e_factor<NT>(e, e.v.G);
   e_bwd_solve_write<NT, TS>(a, e, p);
 }
 
