@@ -26,13 +26,23 @@ __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a 
 // Copies of a lane index or of a uniform size that the compiler cannot hoist out of the
 // iteration loops: lane masks and addresses derived from them are recomputed per stage instead of
 // being kept live in registers across the whole solve.
+// (QPX_PIN_V / QPX_PIN_S and QPX_DYN_LDS are predefined only by the host build of these sources
+// that the CPU bounds tests run.)
+#ifndef QPX_PIN_V
+#define QPX_PIN_V(x) asm volatile("" : "+v"(x))
+#define QPX_PIN_S(x) asm volatile("" : "+s"(x))
+#endif
+// The workgroup's dynamic LDS as `unsigned char* name`.
+#ifndef QPX_DYN_LDS
+#define QPX_DYN_LDS(name) extern __shared__ __align__(16) unsigned char name[]
+#endif
 __device__ __forceinline__ int opaque_v(int x) {
-  asm volatile("" : "+v"(x));
+  QPX_PIN_V(x);
   return x;
 }
 __device__ __forceinline__ int opaque_s(int x) {
   x = __builtin_amdgcn_readfirstlane(x);
-  asm volatile("" : "+s"(x));
+  QPX_PIN_S(x);
   return x;
 }
 
@@ -682,7 +692,7 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
     for (int k = tid; k < mi; k += NT) v.D[k] = dmode == kDunit ? 1.0 : v.z[k] / v.s[k];
   __syncthreads();
   if (n >= kMfmaMinN && n <= kMfmaMaxN) {
-    extern __shared__ __align__(16) unsigned char qpx_smem[];
+    QPX_DYN_LDS(qpx_smem);
     gtdg_mfma<NT>(v, reinterpret_cast<double*>(qpx_smem), G);
   } else {
     for (int e = tid; e < n * n; e += NT) {
@@ -696,7 +706,7 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
   __syncthreads();
   const bool tiled = n >= kMfmaMinN && n <= kMfmaMaxN;
   if (tiled) {
-    extern __shared__ __align__(16) unsigned char qpx_smem[];
+    QPX_DYN_LDS(qpx_smem);
     block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, reinterpret_cast<double*>(qpx_smem));
   } else {
     block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
@@ -707,7 +717,7 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
   if (NT >= 256 && w_in_lds(n, me)) {
     // W in LDS (aliasing the finished G^T D G chunk and Cholesky panel), its trailing updates and
     // F = W^T W on the matrix cores; W is copied to the workspace for the solves
-    extern __shared__ __align__(16) unsigned char qpx_smem[];
+    QPX_DYN_LDS(qpx_smem);
     double* Wl = reinterpret_cast<double*>(qpx_smem);
     __syncthreads();
     for (int e = tid; e < n * me; e += NT) {

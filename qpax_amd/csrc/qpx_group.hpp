@@ -139,7 +139,7 @@ __device__ __forceinline__ double row16_sum(double v) {
 
 template <typename TS, int NTI, int W>
 __device__ __forceinline__ GCtx<TS, NTI, W> make_gctx(int n, int me, int mi, double* ws) {
-  extern __shared__ __align__(16) unsigned char qpx_smem[];
+  QPX_DYN_LDS(qpx_smem);
   constexpr int NT = 64 * W, TT = NTI * (NTI + 1) / 2, SMAX = smax_of(NTI, W);
   GCtx<TS, NTI, W> c;
   c.n = n; c.np = wave::padded_n(n); c.me = me; c.mi = mi; c.m = c.np + me;

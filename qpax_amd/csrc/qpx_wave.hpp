@@ -1134,11 +1134,13 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   WSYNC();
   if (l < n) pa += gcol_dot(c.Gs, n, mi, l, c.vb);
   if (l >= m) pa = 0.0;
-  // forward: unit-lower L y = p; lane l reads row l of L (rows >= m read row m-1, unused)
-  // The factor entries load 16 at a time ahead of the dependency chain.  Reads past the end of
-  // a row run into the following rows or the panel buffer behind the factor (inside this
-  // wavefront's LDS image) and are discarded by the select; lanes of rows >= m stay finite and
-  // only ever meet zero coefficients.
+  // forward: unit-lower L y = p; lane l reads row lr = min(l, m-1) of L.  The factor entries
+  // load 16 at a time ahead of the dependency chain.  Reads past the end of a row run into the
+  // following rows or the panel buffer behind the factor (inside this wavefront's LDS image) and
+  // are discarded by the select, which keeps only columns j < lr: the factorisation writes no
+  // diagonal slot, so lanes of rows >= m (which read row m-1) must not take L[m-1][m-1] either --
+  // their values stay finite whatever the LDS held before, and lanes j >= m only ever meet zero
+  // coefficients.
   constexpr int NB = NMAX / 16;
   const int lr = l < m ? l : m - 1;
   const double* Lrow = c.Lp + lr * (lr + 1) / 2;
@@ -1156,7 +1158,7 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int j = 16 * b + u;
-        pa -= (l > j && j < m ? lv[u] : 0.0) * rl(pa, j);
+        pa -= (lr > j ? lv[u] : 0.0) * rl(pa, j);
       }
       if (b + 1 < NB) {
 #pragma unroll
@@ -1378,7 +1380,7 @@ __device__ __forceinline__ int relax_iter(const Ctx<TS, NMAX, MIC>& c, St<TS, NM
 // a disjoint LDS image of lds_per_wave bytes.
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ Ctx<TS, NMAX, MIC> make_ctx(int n, int me, int mi, double* ws, int lds_per_wave) {
-  extern __shared__ __align__(16) unsigned char qpx_smem[];
+  QPX_DYN_LDS(qpx_smem);
   const Lds w = lds_layout(n, me, mi, (int)sizeof(TS));
   unsigned char* base = qpx_smem + (threadIdx.x >> 6) * lds_per_wave;
   Ctx<TS, NMAX, MIC> c;
