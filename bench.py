@@ -8,8 +8,8 @@ backward) is timed for --steps steps; the plain per-stage path is timed on --pla
 the fused/plain ratio.  Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line (rank 0).  Extra objects: "roofline" for the dominant kernel (fused solve +
-relax) from HIP events on the launch stream, and "cpu_baseline" (the float64 numpy oracle on a
-bounded sample of the same workload, single-threaded).
+relax) from HIP events on the launch stream, and "cpu_baseline" (the float64 numpy oracle on the
+same workload, one single-threaded process per host core, with the CPU model and core counts).
 """
 
 from __future__ import annotations
@@ -118,19 +118,62 @@ def kernel_times(torch, qpax_amd, ts, reps):
     return float(np.mean(f_ms)), float(np.mean(b_ms)), out
 
 
-def cpu_baseline(data, sample, xg_gpu):
-    """float64 numpy oracle, one thread, on the first `sample` problems: fwd (solve + relax) + bwd."""
+_CPU_DATA = None  # the batch, inherited by the forked workers of cpu_baseline
+
+
+def _cpu_chunk(rng):
+    """Oracle fwd (solve + relax) + bwd on problems rng[0]:rng[1] of _CPU_DATA (one worker)."""
     from threadpoolctl import threadpool_limits
 
     from oracle import qp_oracle as O
 
-    d64 = [v[:sample].astype(np.float64) for v in data]
+    lo, hi = rng
+    d64 = [v[lo:hi].astype(np.float64) for v in _CPU_DATA]
     with threadpool_limits(1):
-        t0 = time.perf_counter()
         xo, rel, info = O.solve_qp_primal_fwd(*d64, solver_tol=TOL, target_kappa=KAPPA)
         go = O.solve_qp_primal_bwd(*d64, rel, 2 * (xo - 1))
+    return xo, go, info
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, affinity
+
+
+def cpu_baseline(data, sample):
+    """float64 numpy oracle on the first `sample` problems, fwd (solve + relax) + bwd, split over
+    one forked worker per host core of this process's affinity (at most 16, the GPU's share of the
+    host).  Runs before anything touches the GPU, so the workers are plain forks."""
+    import multiprocessing as mp
+
+    global _CPU_DATA
+    model, nproc, affinity = host_cpu()
+    workers = max(1, min(16, affinity, sample))
+    _CPU_DATA = data
+    bounds = np.linspace(0, sample, workers + 1).astype(int)
+    chunks = [(int(a), int(b)) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+    with mp.get_context("fork").Pool(len(chunks)) as pool:
+        pool.map(_cpu_chunk, [(a, min(a + 1, b)) for a, b in chunks])  # warm-up (imports)
+        t0 = time.perf_counter()
+        parts = pool.map(_cpu_chunk, chunks)
         dt = time.perf_counter() - t0
-    return sample / dt, xo, go, info
+    _CPU_DATA = None
+    xo = np.concatenate([p[0] for p in parts])
+    go = [np.concatenate([p[1][k] for p in parts]) for k in range(6)]
+    info = [np.concatenate([p[2][k] for p in parts]) for k in range(4)]
+    host = {"cpu_model": model, "nproc": nproc, "affinity_cores": affinity, "workers": len(chunks)}
+    return sample / dt, xo, go, info, host
 
 
 def uneven_bench(torch, args):
@@ -273,7 +316,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only run a few fused steps (for rocprofv3)")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a plain device copy (counter calibration)")
@@ -288,6 +331,14 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
         sys.exit(2)
+
+    cpu = None
+    if args.workload == "c3" and not args.no_cpu and not args.profile and not args.copy_calib:
+        # the CPU baseline first, before this process initialises the GPU (its workers are forks)
+        from qpax_amd import problems as P
+
+        data = [np.asarray(v, np.float32) for v in P.random_feasible(BATCH, N, ME, MI, seed=SEED)]
+        cpu = cpu_baseline(data, min(args.cpu_sample, BATCH))
 
     import torch
 
@@ -370,9 +421,9 @@ def main():
             "algorithmic_flops_per_launch": fl_fwd,
         },
     }
-    if not args.no_cpu:
-        cpu_qps, xo, go, info = cpu_baseline(data, args.cpu_sample, None)
-        S = args.cpu_sample
+    if cpu is not None:
+        cpu_qps, xo, go, info, host = cpu
+        S = xo.shape[0]
         ok = (info[0] == 1) & (info[2] == 1)
 
         def rel(a, b):
@@ -385,9 +436,10 @@ def main():
         res["x_rel_err_vs_f64_oracle"] = {"max_converged": float(ex[ok].max()), "median": float(np.median(ex)),
                                           "oracle_converged_frac": float(ok.mean()), "sample": S}
         res["grad_rel_err_vs_f64_oracle"] = {k: float(e[ok].max()) for k, e in zip("Q q A b G h".split(), eg)}
-        res["cpu_baseline"] = {"value": round(cpu_qps, 2), "unit": "QPs/s", "cores": 1, "kind": "port",
-                               "sample": f"first {S} of the 4096 QPs, fwd (solve+relax) + bwd, float64 numpy oracle "
-                                         f"(QR solves), 1 thread"}
+        res["cpu_baseline"] = {"value": round(cpu_qps, 2), "unit": "QPs/s", "cores": host["workers"], "kind": "port",
+                               "sample": f"{S} of the 4096 QPs, fwd (solve+relax) + bwd, float64 numpy oracle "
+                                         f"(QR solves), one single-threaded process per core",
+                               **host}
     print(json.dumps(res), flush=True)
 
 

@@ -39,6 +39,7 @@ struct Args {
   unsigned* qhead;        // work-queue head for persistent launches (null: one problem per slot)
   int lds_per_wave;       // bytes of LDS per wavefront (wavefront-per-QP kernels)
   int64_t split_fused;    // split forward launch: problems whose relaxation runs in the solve's item
+  int split_spin;         // split forward launch: polls of a relaxation's wait for its solve
 };
 
 // Sizes and slice offsets of problem p.

@@ -31,6 +31,9 @@ constexpr int kMaxQueues = 512;  // queue heads at words 1..kMaxQueues of the co
 int64_t stride_words(int n, int me, int mi);
 int max_lds_per_block();
 int cu_count();
+// Polls a split relaxation waits for its solve: `dflt`, or the environment variable
+// QPX_SPLIT_SPIN when set (a test lowers it to 0 to make every split wait give up).
+int split_spin_limit(int dflt);
 
 // Wavefront-per-QP kernel instance (NMAX, MIC) for a problem, or false if it does not fit one.
 struct WaveVariant {

@@ -9,6 +9,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -481,6 +482,13 @@ int max_lds_per_block() {
     v = val;
   }
   return v;
+}
+
+int split_spin_limit(int dflt) {
+  const char* v = std::getenv("QPX_SPLIT_SPIN");
+  if (!v || !*v) return dflt;
+  const long r = std::strtol(v, nullptr, 10);
+  return r < 0 ? 0 : (r > dflt ? dflt : (int)r);
 }
 
 int cu_count() {

@@ -1440,9 +1440,10 @@ __device__ __forceinline__ int64_t next_problem(const Args<TS>& a, int& round) {
 // matrix in its workspace slice and raises a ready flag (release); the relaxation of the same
 // problem waits for the flag (acquire), reloads G into LDS and continues from that state.  A
 // relaxation's ticket is taken after its solve's, which a running wavefront holds, so the wait
-// ends; the wait is bounded all the same (after kSplitSpin polls the relaxation reports
-// riters = -1 and stops).  The last work items of the launch are then relaxations (about 4
-// iterations) instead of whole problems, and only the last problems pay the second load.
+// ends; the wait is bounded all the same (after a.split_spin polls, kSplitSpin unless the host
+// lowers it, the relaxation writes NaN to its relaxed point, reports riters = -1 and stops).  The
+// last work items of the launch are then relaxations (about 4 iterations) instead of whole
+// problems, and only the last problems pay the second load.
 constexpr int kSplitSpin = 1 << 22;
 
 template <typename TS, int NMAX, int MIC>
@@ -1460,7 +1461,7 @@ __device__ __forceinline__ void wave_split(const Args<TS>& a, int64_t F) {
   const int l = c.l;
   if (rel) {
     bool ok = false;
-    for (int t = 0; t < kSplitSpin; ++t) {
+    for (int t = 0; t < a.split_spin; ++t) {
       if (__hip_atomic_load(ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
         ok = true;
         break;
@@ -1468,6 +1469,14 @@ __device__ __forceinline__ void wave_split(const Args<TS>& a, int64_t F) {
       __builtin_amdgcn_s_sleep(8);
     }
     if (!ok) {
+      // no relaxed point: NaN, so that nothing downstream (the backward) takes garbage for one
+      const TS nan = (TS)NAN;
+      if (l < pi.n) a.xr[pi.oq + l] = nan;
+      if (l < pi.me) a.yr[pi.ob + l] = nan;
+      for (int k = l; k < pi.mi; k += 64) {
+        a.sr[pi.oh + k] = nan;
+        a.zr[pi.oh + k] = nan;
+      }
       if (l == 0) { a.rconv[p] = 0; a.riters[p] = -1; }
       return;
     }

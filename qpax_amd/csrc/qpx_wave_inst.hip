@@ -17,6 +17,7 @@ hipError_t launch_wave_solve(const Args<TS>& a, unsigned grid, int wpb, hipStrea
     b.qhead = a.counter + 1;
     const int64_t split = 8 * (int64_t)qpx::host::cu_count();
     b.split_fused = a.B > split ? a.B - split : 0;
+    b.split_spin = qpx::host::split_spin_limit(kSplitSpin);
     const int64_t flag_word = split_state_off(a.n, a.me) + kSplitStateWords;
     hipLaunchKernelGGL((k_wave_split_clear<TS>), dim3((unsigned)((a.B + 255) / 256)), dim3(256), 0, st, b, flag_word);
     hipLaunchKernelGGL((k_wave_solve<TS, NMAX, MIC, true>), dim3((unsigned)(2 * a.B - b.split_fused)), dim3(64),

@@ -65,7 +65,20 @@ def solve_qp_relaxed_batch(bt: Batch, solver_tol=1e-5, target_kappa=1e-3, mode="
                                   ptr(xr), ptr(sr), ptr(zr), ptr(yr), ptr(flags[2]), ptr(flags[3]),
                                   ptr(ws), nbytes, stream_of(bt.device))
     _lib.check(st, "qpx_solve_qp_relaxed")
+    check_relaxed(flags[3])
     return x, s, z, y, flags[0], flags[1], xr, sr, zr, yr, flags[2], flags[3]
+
+
+def check_relaxed(riters):
+    """Raises if a relaxation reported riters = -1: its wait for its own solve gave up (the library
+    then writes NaN to that problem's relaxed point).  Reads the flags back, so it is skipped while
+    a graph is being captured."""
+    if riters.numel() == 0 or torch.cuda.is_current_stream_capturing():
+        return
+    bad = int((riters < 0).sum())
+    if bad:
+        raise RuntimeError(f"qpx_solve_qp_relaxed: the relaxation of {bad} problem(s) did not start (riters = -1); "
+                           "their relaxed points are NaN")
 
 
 def solve_qp(Q, q, A, b, G, h, solver_tol: float = 1e-3, mode: str = "fused"):

@@ -4,7 +4,7 @@ random feasible QPs (n=20, m_eq=5, m_ineq=40) in float32 and float64 at solver_t
 default of solve_qp (qpax/pdip.py:181).
 
 Every problem of each batch is checked against the float64 oracle (flags identical, iteration
-counts within one, x within the dtype's bar), and, independently of the oracle, through properties
+counts within one, x, s, z and y within the dtype's bar), and, independently of the oracle, through properties
 that hold whatever the batch size:
   - a converged problem's returned iterate has s > 0, z > 0 and a KKT residual below the tolerance
     (the reference takes one more step after the residual test passes, pdip.py:87-119, which
@@ -64,10 +64,11 @@ def test_full_size_forward(name, gen, dt, tol, xtol, mode):
     assert ok.mean() > 0.99
     np.testing.assert_array_equal(conv, co)
     assert np.abs(iters - io).max() <= 1
-    ex = rel_err(x[ok], xo[ok])
-    print(f"{name} {mode}: x rel err max {ex.max():.2e} median {np.median(ex):.2e}; "
-          f"iters mean {iters.mean():.2f} max {iters.max()}; converged {conv.mean():.4f}")
-    assert ex.max() < xtol
+    errs = {k: rel_err(a[ok], b[ok]).max() for k, a, b in zip("xszy", (x, s, z, y), (xo, so, zo, yo)) if b.shape[-1]}
+    print(f"{name} {mode}: " + ", ".join(f"{k} rel err max {v:.2e}" for k, v in errs.items())
+          + f"; iters mean {iters.mean():.2f} max {iters.max()}; converged {conv.mean():.4f}")
+    for k, v in errs.items():
+        assert v < xtol, (k, v)
 
     # properties of the returned iterate, computed from the GPU outputs alone
     c = conv == 1

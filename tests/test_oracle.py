@@ -354,3 +354,17 @@ def test_elastic_equality_extension_grads_match_expanded(consistent):
     assert ok.mean() > 0.5
     for a, r in zip(ge, ref):
         assert np.abs(a - r)[ok].max() < 1e-3 * max(1.0, np.abs(r[ok]).max())
+
+
+def test_conditioning_floor_separates_well_and_ill_conditioned_batches():
+    """tests._util.conditioning_floor (the float64 bar of the GPU state tests) stays far below 1e-9
+    on well-conditioned problems, so there the bar is the fixed 1e-9, and rises above it on
+    problems whose KKT systems are ill-conditioned at the stopping point (n=52, m_ineq=256)."""
+    from tests._util import conditioning_floor
+
+    for (n, me, mi, seed, tol), lo, hi in (((20, 5, 40, 2, 1e-3), 0.0, 1e-9), ((52, 8, 256, 39, 1e-6), 1e-9, 1e-6)):
+        data = list(P.random_feasible(16, n, me, mi, seed=seed))
+        ref = O.solve_qp(*data, solver_tol=tol)
+        fl = conditioning_floor(lambda *v: O.solve_qp(*v, solver_tol=tol), data, ref, ref[4] == 1)
+        worst = max(fl.values())
+        assert lo <= worst < hi, (n, fl)
