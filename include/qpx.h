@@ -16,7 +16,9 @@
  *
  * Modes: QPX_PLAIN runs one kernel per IPM stage per iteration and checks convergence on the host
  * (one device->host read per iteration); QPX_FUSED runs the whole loop in one kernel launch per
- * phase with no host synchronisation (capturable in a hipGraph).  Both give identical results.
+ * phase with no host synchronisation (capturable in a hipGraph).  The two modes run different
+ * eliminations for the problems the fused kernels take, so their results agree to rounding (x within
+ * about 1e-9 relative in float64 on the test problems), not bit for bit.
  */
 #ifndef QPX_H_
 #define QPX_H_
@@ -44,8 +46,10 @@ enum { QPX_PLAIN = 0, QPX_FUSED = 1 };
 enum {
   QPX_EXACT_DG = 1,           /* dG = diag(lam) dlam x^T + lam dx^T, the exact VJP (default: the
                                  reference's diag(lam) (dlam x^T + lam dx^T), qpax/diff_qp.py:11) */
-  QPX_ELASTIC_CONSISTENT = 2  /* elastic relaxation with the forward's signs of t and r2 (default:
+  QPX_ELASTIC_CONSISTENT = 2, /* elastic relaxation with the forward's signs of t and r2 (default:
                                  the reference's flipped signs, qpax/elastic_qp.py:257-261) */
+  QPX_TRACE_DEBUG_STEP = 4    /* qpx_solve_qp_trace: the step length rule of the reference's debug
+                                 loop, 0.99 min(1, 0.99 ort(s), 0.99 ort(z)) (qpax/pdip.py:281) */
 };
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -106,6 +110,27 @@ QPX_API qpx_status qpx_solve_qp_special(int32_t dtype, int64_t batch, int32_t n,
                                         void* workspace, size_t workspace_bytes, void* stream);
 QPX_API size_t qpx_special_workspace_size(int64_t batch, int32_t n, int32_t m_eq);
 
+/* qpax.pdip.initialize -- replaces qpax/pdip.py:15-35 (the CVXOPT initialisation: one reduced KKT
+ * solve with D = 1 and the shift of s, z into the positive orthant), on the problem as solve_qp
+ * hands it over: Q symmetrised, rows with infinite h zeroed (pdip.py:220-223).  Outputs x (B,n),
+ * s, z (B,m_ineq), y (B,m_eq).  Workspace: qpx_workspace_size(batch, n, m_eq, m_ineq). */
+QPX_API qpx_status qpx_initialize(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+                                  const void* Q, const void* q, const void* A, const void* b, const void* G,
+                                  const void* h, void* x, void* s, void* z, void* y, void* workspace,
+                                  size_t workspace_bytes, void* stream);
+
+/* qpax.solve_qp with a per-iteration trace -- the device-side counterpart of solve_qp_debug
+ * (qpax/pdip.py:254-360), which prints one line per predictor-corrector iteration.  Solves as
+ * qpx_solve_qp (the workgroup-per-QP kernels) and writes trace (B, 30, 5) float64: for iteration k,
+ * the infinity norms of r1, r2, r3, r4 at the iteration's entry (r4: 0 without equality rows) and
+ * the step length taken; rows of iterations not taken are NaN.  flags: 0 (the step rule of
+ * solve_qp, pdip.py:112) or QPX_TRACE_DEBUG_STEP (the debug loop's rule, pdip.py:281). */
+QPX_API qpx_status qpx_solve_qp_trace(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+                                      const void* Q, const void* q, const void* A, const void* b, const void* G,
+                                      const void* h, double solver_tol, int32_t flags, void* x, void* s, void* z,
+                                      void* y, int32_t* converged, int32_t* iters, double* trace, void* workspace,
+                                      size_t workspace_bytes, void* stream);
+
 /* Elastic QP:  minimise 1/2 x'Qx + q'x + penalty 1't  s.t.  G x - t <= h, t >= 0 [, A x = b].
  * qpx_solve_qp_elastic replaces qpax.solve_qp_elastic (qpax/elastic_qp.py:184-228: the
  * initialisation :9-47 and the predictor-corrector loop of Algorithm 4 :99-174 with the
@@ -131,6 +156,15 @@ QPX_API qpx_status qpx_solve_qp_elastic_relaxed(
     void* s1, void* s2, void* z1, void* z2, void* y, int32_t* converged, int32_t* iters, void* xr,
     void* tr, void* s1r, void* s2r, void* z1r, void* z2r, void* yr, int32_t* relax_converged,
     int32_t* relax_iters, void* workspace, size_t workspace_bytes, void* stream);
+
+/* qpax.elastic_qp.initialize_elastic -- replaces qpax/elastic_qp.py:9-47 (solve_init_elastic_ls and
+ * the CVXOPT shift on the stacked (z1, z2)), with Q symmetrised as solve_qp_elastic hands it over
+ * (:186).  Outputs x (B,n), t, s1, s2, z1, z2 (B,m_ineq) [, y (B,m_eq) with equality rows]. */
+QPX_API qpx_status qpx_initialize_elastic(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+                                          const void* Q, const void* q, const void* A, const void* b, const void* G,
+                                          const void* h, const void* penalty, void* x, void* t, void* s1, void* s2,
+                                          void* z1, void* z2, void* y, void* workspace, size_t workspace_bytes,
+                                          void* stream);
 
 /* Backward of qpax.solve_qp_elastic_primal -- replaces solve_qp_elastic_primal_bwd /
  * diff_qp_elastic / optnet_derivatives_elastic (qpax/elastic_qp.py:366-417, :459-466):

@@ -244,6 +244,65 @@ def solve_qp(Q, q, A, b, G, h, solver_tol=1e-3):
     return x, s, z, y, conv, iters
 
 
+def solve_qp_trace(Q, q, A, b, G, h, solver_tol=1e-3, debug_step=False):
+    """solve_qp with the per-iteration table of solve_qp_debug (qpax/pdip.py:254-360): returns
+    ((x, s, z, y, converged, iters), trace) with trace[:, k] = (|r1|, |r2|, |r3|, |r4|, alpha) of
+    iteration k (infinity norms at the iteration's entry, |r4| = 0 without equality rows as the
+    debug print's zeros(1), pdip.py:285-286) and NaN rows after the last iteration.  debug_step
+    takes the debug loop's step rule 0.99 min(1, 0.99 ort(s), 0.99 ort(z)) (pdip.py:281) instead
+    of solve_qp's 0.99 min(ort(s), ort(z)) (pdip.py:112)."""
+    Q, q, A, b, G, h = _as_batch(Q, q, A, b, G, h)
+    B = q.shape[0]
+    G, h, mask = remove_inf_constraints(G, h)
+    Q = 0.5 * (Q + np.swapaxes(Q, -1, -2))
+    x, s, z, y = initialize(Q, q, A, b, G, h)
+    trace = np.full((B, MAX_ITERS, 5), np.nan)
+
+    def norm(v):
+        return np.max(np.abs(v), axis=-1) if v.shape[-1] else np.zeros(v.shape[0])
+
+    def step(Q, q, A, b, G, h, x, s, z, y):
+        r1 = _mv(Q, x) + q + _mtv(A, y) + _mtv(G, z)
+        r2 = s * z
+        r3 = _mv(G, x) + s - h
+        r4 = _mv(A, x) - b
+        rows = np.stack([norm(r1), norm(r2), norm(r3), norm(r4)], axis=-1)
+        converged = np.where(_inf_norm(r1, r2, r3, r4) < solver_tol, 1, 0).astype(np.int32)
+        P_inv_vec, L_H, L_F = factorize_kkt(Q, G, A, s, z)
+        _, ds_a, dz_a, _ = solve_kkt_rhs(Q, G, A, s, z, P_inv_vec, L_H, L_F, -r1, -r2, -r3, -r4)
+        sigma, mu = centering_params(s, z, ds_a, dz_a)
+        r2 = r2 - ((sigma * mu)[..., None] - (ds_a * dz_a))
+        dx, ds, dz, dy = solve_kkt_rhs(Q, G, A, s, z, P_inv_vec, L_H, L_F, -r1, -r2, -r3, -r4)
+        a_s, a_z = ort_linesearch(s, ds), ort_linesearch(z, dz)
+        if debug_step:
+            alpha = 0.99 * np.minimum(np.minimum(1.0, 0.99 * a_s), 0.99 * a_z)
+        else:
+            alpha = 0.99 * np.minimum(a_s, a_z)
+        rows = np.concatenate([rows, alpha[..., None]], axis=-1)
+        a = alpha[..., None]
+        return x + a * dx, s + a * ds, z + a * dz, y + a * dy, rows, converged
+
+    conv = np.zeros(B, np.int32)
+    iters = np.zeros(B, np.int32)
+    state = [x, s, z, y]
+    consts = [Q, q, A, b, G, h]
+    while True:
+        act = np.nonzero((iters < MAX_ITERS) & (conv == 0))[0]
+        if act.size == 0:
+            break
+        with np.errstate(all="ignore"):
+            out = step(*[c[act] for c in consts], *[v[act] for v in state])
+        for v, nv in zip(state, out[:4]):
+            v[act] = nv
+        trace[act, iters[act]] = out[4]
+        conv[act] = out[5]
+        iters[act] += 1
+    x, s, z, y = state
+    z = np.where(mask, 0, z)
+    s = np.where(mask, np.inf, s)
+    return (x, s, z, y, conv, iters), trace
+
+
 # ----------------------------------------------------------------------------------------------
 # relaxation and implicit backward (qpax/pdip_relaxed.py, qpax/diff_qp.py)
 # ----------------------------------------------------------------------------------------------

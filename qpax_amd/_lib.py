@@ -36,6 +36,10 @@ SIGNATURES = {
     "qpx_diff_qp_elastic": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_vp] * 8 + [_i32] + [_vp] * 6
                             + [_vp, _sz, _vp]),
     "qpx_solve_qp_special": (_i32, [_i32, _i64, _i32, _i32] + [_vp] * 4 + [_vp] * 4 + [_vp, _sz, _vp]),
+    "qpx_initialize": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_vp] * 4 + [_vp, _sz, _vp]),
+    "qpx_initialize_elastic": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_vp] * 7 + [_vp, _sz, _vp]),
+    "qpx_solve_qp_trace": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_f64, _i32] + [_vp] * 7
+                           + [_vp, _sz, _vp]),
     "qpx_special_workspace_size": (_sz, [_i64, _i32, _i32]),
     "qpx_plan_create": (_i32, [_i32, _i64, _vp, _vp, _vp, _vp]),
     "qpx_plan_destroy": (None, [_vp]),
@@ -48,7 +52,8 @@ SIGNATURES = {
     "qpx_plan_diff": (_i32, [_vp] + [_vp] * 11 + [_i32] + [_vp] * 6 + [_vp, _sz, _vp]),
     "qpx_plan_tune": (_i32, [_vp] + [_vp] * 6 + [_f64, _f64] + [_vp] * 12 + [_vp, _sz, _vp]),
 }
-QPX_EXACT_DG, QPX_ELASTIC_CONSISTENT = 1, 2
+QPX_EXACT_DG, QPX_ELASTIC_CONSISTENT, QPX_TRACE_DEBUG_STEP = 1, 2, 4
+TRACE_ITERS, TRACE_COLS = 30, 5
 
 _lib = None
 

@@ -40,7 +40,10 @@ struct Args {
   int lds_per_wave;       // bytes of LDS per wavefront (wavefront-per-QP kernels)
   int64_t split_fused;    // split forward launch: problems whose relaxation runs in the solve's item
   int split_spin;         // split forward launch: polls of a relaxation's wait for its solve
+  double* trace;          // per-problem iteration trace (kMaxIters rows of kTraceCols), or null
 };
+
+constexpr int kTraceCols = 5;  // |r1|, |r2|, |r3|, |r4| (infinity norms at entry), step length
 
 // Sizes and slice offsets of problem p.
 struct PIdx {

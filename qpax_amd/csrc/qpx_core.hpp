@@ -835,10 +835,12 @@ __device__ void stage_solve(const View& v, const double* a1, const double* a2, c
 
 // --------------------------------------------------------------------------------------------
 // Stage: residuals of the (relaxed) KKT system and the entry convergence flag
-// (qpax/pdip.py:93-100; pdip_relaxed.py:16-23 with kappa).  Returns the flag.
+// (qpax/pdip.py:93-100; pdip_relaxed.py:16-23 with kappa).  Returns the flag.  With `norms`,
+// thread 0 also writes the infinity norms of r1, r2, r3 and r4 there (0 for an empty block, as
+// the reference's debug table prints, pdip.py:286-298).
 // --------------------------------------------------------------------------------------------
 template <int NT>
-__device__ int stage_residuals(const View& v, double kappa, double tol, double* red) {
+__device__ int stage_residuals(const View& v, double kappa, double tol, double* red, double* norms = nullptr) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   const bool wide = n >= kMfmaMinN;
   if (wide) {  // G x into r3 and A x into r4 first; Q x by columns (Q is symmetric), coalesced
@@ -846,7 +848,7 @@ __device__ int stage_residuals(const View& v, double kappa, double tol, double* 
     rows_dot16<NT>(v.A, me, n, v.x, v.r4);
     __syncthreads();
   }
-  double m = 0.0;
+  double m1 = 0.0, m2 = 0.0, m3 = 0.0, m4 = 0.0;
   for (int i = tid; i < n; i += NT) {
     double acc = 0.0;
     if (wide)
@@ -857,7 +859,7 @@ __device__ int stage_residuals(const View& v, double kappa, double tol, double* 
     for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
     for (int k = 0; k < mi; ++k) acc += v.G[k * n + i] * v.z[k];
     v.r1[i] = acc;
-    m = nmax(m, fabs(acc));
+    m1 = nmax(m1, fabs(acc));
   }
   for (int k = tid; k < mi; k += NT) {
     double acc = 0.0;
@@ -868,7 +870,8 @@ __device__ int stage_residuals(const View& v, double kappa, double tol, double* 
     const double r2 = v.s[k] * v.z[k] - kappa;
     v.r2[k] = r2;
     v.r3[k] = r3;
-    m = nmax(m, nmax(fabs(r2), fabs(r3)));
+    m2 = nmax(m2, fabs(r2));
+    m3 = nmax(m3, fabs(r3));
   }
   for (int c = tid; c < me; c += NT) {
     double acc = 0.0;
@@ -877,9 +880,22 @@ __device__ int stage_residuals(const View& v, double kappa, double tol, double* 
       for (int j = 0; j < n; ++j) acc += v.A[c * n + j] * v.x[j];
     const double r4 = acc - v.b[c];
     v.r4[c] = r4;
-    m = nmax(m, fabs(r4));
+    m4 = nmax(m4, fabs(r4));
   }
-  m = block_reduce<NT>(m, red, OpMax());
+  double m;
+  if (norms) {
+    m1 = block_reduce<NT>(m1, red, OpMax());
+    __syncthreads();
+    m2 = block_reduce<NT>(m2, red, OpMax());
+    __syncthreads();
+    m3 = block_reduce<NT>(m3, red, OpMax());
+    __syncthreads();
+    m4 = block_reduce<NT>(m4, red, OpMax());
+    if (tid == 0) { norms[0] = m1; norms[1] = m2; norms[2] = m3; norms[3] = m4; }
+    m = nmax(nmax(m1, m2), nmax(m3, m4));
+  } else {
+    m = block_reduce<NT>(nmax(nmax(m1, m2), nmax(m3, m4)), red, OpMax());
+  }
   __syncthreads();
   return (m < tol) ? 1 : 0;
 }
@@ -917,7 +933,7 @@ __device__ void stage_center(const View& v, double* red) {
 // Stage: line search and update (qpax/pdip.py:112-117 for the predictor-corrector step,
 // pdip_relaxed.py:30-35 for the relaxed Newton step).
 template <int NT>
-__device__ void stage_step(const View& v, bool relax_rule, double* red) {
+__device__ double stage_step(const View& v, bool relax_rule, double* red) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   const double as = block_ort<NT>(v.s, v.ds, mi, red);
   const double az = block_ort<NT>(v.z, v.dz, mi, red);
@@ -926,6 +942,7 @@ __device__ void stage_step(const View& v, bool relax_rule, double* red) {
   for (int k = tid; k < mi; k += NT) { v.s[k] += alpha * v.ds[k]; v.z[k] += alpha * v.dz[k]; }
   for (int c = tid; c < me; c += NT) v.y[c] += alpha * v.dy[c];
   __syncthreads();
+  return alpha;
 }
 
 // Stage: initialisation (qpax/pdip.py:15-35).  Factor with D = 1, solve the reduced system with
@@ -962,15 +979,19 @@ __device__ void stage_init(const View& v, double* red) {
   __syncthreads();
 }
 
-// One predictor-corrector iteration (qpax/pdip.py:87-119) on an active problem.
+// One predictor-corrector iteration (qpax/pdip.py:87-119) on an active problem.  With `trace`,
+// thread 0 writes the entry residual norms (r1, r2, r3, r4) and the step length there;
+// `debug_step` takes the step length rule of the reference's debug loop,
+// 0.99 min(1, 0.99 ort(s), 0.99 ort(z)) (pdip.py:281), instead of 0.99 min(ort(s), ort(z)).
 template <int NT>
-__device__ void pc_iteration(const View& v, double tol, double* red) {
-  const int conv = stage_residuals<NT>(v, 0.0, tol, red);
+__device__ void pc_iteration(const View& v, double tol, double* red, double* trace = nullptr, bool debug_step = false) {
+  const int conv = stage_residuals<NT>(v, 0.0, tol, red, trace);
   stage_factor<NT>(v, kDzs);
   stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.dsa, v.dza);
   stage_center<NT>(v, red);
   stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
-  stage_step<NT>(v, false, red);
+  const double alpha = stage_step<NT>(v, debug_step, red);
+  if (trace && threadIdx.x == 0) trace[4] = alpha;
   if (threadIdx.x == 0) { v.ctl[kConv] = conv; v.ctl[kIter] += 1; }
   __syncthreads();
 }

@@ -325,6 +325,17 @@ __global__ __launch_bounds__(NT) void k_e_setup(Args<TS> a) {
   e_init<NT>(e, red);
 }
 
+// initialize_elastic (qpax/elastic_qp.py:9-47) alone, written to x, t, s1, s2, z1, z2 (and y).
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_e_initialize(Args<TS> a) {
+  __shared__ double red[NT / 64];
+  const int64_t p = blockIdx.x;
+  EView e = eview_of(a, p);
+  QPX_E_LOAD(NT, TS, e, a, p);
+  e_init<NT>(e, red);
+  e_write_state<NT, TS>(e, p, a.x, a.t, a.s, a.s2, a.z, a.z2, a.y);
+}
+
 // Plain mode: one kernel per iteration-stage group (residuals, factor, solves, step).
 template <int NT, typename TS>
 __global__ __launch_bounds__(NT) void k_e_resid(Args<TS> a, int relax) {

@@ -3,8 +3,10 @@
 // One thread block owns one QP.  The plain mode launches one kernel per IPM stage per iteration
 // over the whole padded batch and reads the number of still-active problems back to the host
 // after every iteration; the fused mode runs each phase (solve + relax, backward) as a single
-// launch in which every block iterates its own problem to completion.  Both modes call the same
-// device stages in the same order, so they produce bit-identical results.
+// launch in which every block iterates its own problem to completion.  The fused mode of a size
+// that neither the wavefront nor the matrix-core workgroup kernels take calls the same device
+// stages in the same order as the plain mode, bit for bit; for the other sizes the fused kernels
+// run another elimination and agree with the plain mode to rounding.
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -175,9 +177,14 @@ __global__ __launch_bounds__(NT) void k_fused_solve(Args<TS> a, int with_relax) 
     const View v = view_of(a, pi);
     stage_load<NT, TS>(v, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
     stage_init<NT>(v, red);
+    double* tr = a.trace ? a.trace + p * (int64_t)(kMaxIters * kTraceCols) : nullptr;
     while (is_active(v, false)) {
       __syncthreads();
-      pc_iteration<NT>(v, a.tol, red);
+      pc_iteration<NT>(v, a.tol, red, tr ? tr + v.ctl[kIter] * kTraceCols : nullptr, (a.flags & 4) != 0);
+    }
+    if (tr) {  // rows of iterations not taken
+      __syncthreads();
+      for (int e = v.ctl[kIter] * kTraceCols + threadIdx.x; e < kMaxIters * kTraceCols; e += NT) tr[e] = NAN;
     }
     finish_solve<NT, TS>(a, v, p);
     if (!with_relax) continue;
@@ -187,6 +194,22 @@ __global__ __launch_bounds__(NT) void k_fused_solve(Args<TS> a, int with_relax) 
     }
     finish_relax<NT, TS>(a, v, p);
   }
+}
+
+// initialize (qpax/pdip.py:15-35) alone: the load of solve_qp (Q symmetrised, rows with infinite
+// h zeroed, pdip.py:220-223) and the CVXOPT initialisation, written to x, s, z, y.
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_initialize(Args<TS> a) {
+  __shared__ double red[NT / 64];
+  const int64_t p = blockIdx.x;
+  const PIdx pi = pidx(a, p);
+  const View v = view_of(a, pi);
+  stage_load<NT, TS>(v, a.Q + pi.oQ, a.q + pi.oq, a.A + pi.oA, a.b + pi.ob, a.G + pi.oG, a.h + pi.oh);
+  stage_init<NT>(v, red);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < pi.n; i += NT) a.x[pi.oq + i] = (TS)v.x[i];
+  for (int k = tid; k < pi.mi; k += NT) { a.s[pi.oh + k] = (TS)v.s[k]; a.z[pi.oh + k] = (TS)v.z[k]; }
+  for (int c = tid; c < pi.me; c += NT) a.y[pi.ob + c] = (TS)v.y[c];
 }
 
 // ------------------------------------ backward ---------------------------------------------
@@ -961,6 +984,93 @@ QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batc
                                   workspace, stream);
   return e_backward_impl<double>(mode, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, penalty, rel, dl_dx, flags, grads,
                                  workspace, stream);
+}
+
+QPX_API qpx_status qpx_initialize(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+                                  const void* Q, const void* q, const void* A, const void* b, const void* G,
+                                  const void* h, void* x, void* s, void* z, void* y, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  qpx::host::clear_error();
+  qpx_status st = check_dims(dtype, QPX_FUSED, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, x, s, z, workspace})) return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (batch == 0) return QPX_OK;
+  auto launch = [&](auto tag) -> qpx_status {
+    using TS = decltype(tag);
+    auto a = make_args<TS>(batch, n, m_eq, m_ineq, workspace);
+    a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+    a.G = (const TS*)G; a.h = (const TS*)h;
+    a.x = (TS*)x; a.s = (TS*)s; a.z = (TS*)z; a.y = (TS*)y;
+    const int lds = qpx::generic_lds_bytes(n, m_eq);
+    return dispatch_nt(n, m_ineq, [&](auto nt) {
+      hipLaunchKernelGGL((qpx::k_initialize<decltype(nt)::value, TS>), dim3((unsigned)batch), dim3(decltype(nt)::value),
+                         lds, (hipStream_t)stream, a);
+      QPX_HIP(hipGetLastError());
+      return QPX_OK;
+    });
+  };
+  return dtype == QPX_F32 ? launch(float()) : launch(double());
+}
+
+QPX_API qpx_status qpx_initialize_elastic(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+                                          const void* Q, const void* q, const void* A, const void* b, const void* G,
+                                          const void* h, const void* penalty, void* x, void* t, void* s1, void* s2,
+                                          void* z1, void* z2, void* y, void* workspace, size_t workspace_bytes,
+                                          void* stream) {
+  qpx::host::clear_error();
+  qpx_status st = check_dims(dtype, QPX_FUSED, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, penalty, x, t, s1, s2, z1, z2, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (batch == 0) return QPX_OK;
+  auto launch = [&](auto tag) -> qpx_status {
+    using TS = decltype(tag);
+    auto a = make_args<TS>(batch, n, m_eq, m_ineq, workspace);
+    a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+    a.G = (const TS*)G; a.h = (const TS*)h; a.pen = (const TS*)penalty;
+    a.x = (TS*)x; a.t = (TS*)t; a.s = (TS*)s1; a.s2 = (TS*)s2; a.z = (TS*)z1; a.z2 = (TS*)z2; a.y = (TS*)y;
+    const int lds = qpx::generic_lds_bytes(n, m_eq);
+    return dispatch_nt(n, m_ineq, [&](auto nt) {
+      hipLaunchKernelGGL((qpx::k_e_initialize<decltype(nt)::value, TS>), dim3((unsigned)batch),
+                         dim3(decltype(nt)::value), lds, (hipStream_t)stream, a);
+      QPX_HIP(hipGetLastError());
+      return QPX_OK;
+    });
+  };
+  return dtype == QPX_F32 ? launch(float()) : launch(double());
+}
+
+QPX_API qpx_status qpx_solve_qp_trace(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq, int32_t m_ineq,
+                                      const void* Q, const void* q, const void* A, const void* b, const void* G,
+                                      const void* h, double solver_tol, int32_t flags, void* x, void* s, void* z,
+                                      void* y, int32_t* converged, int32_t* iters, double* trace, void* workspace,
+                                      size_t workspace_bytes, void* stream) {
+  qpx::host::clear_error();
+  qpx_status st = check_dims(dtype, QPX_FUSED, batch, n, m_eq, m_ineq, workspace_bytes);
+  if (st != QPX_OK) return st;
+  if (batch > 0 && any_null({Q, q, G, h, x, s, z, converged, iters, trace, workspace}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y})) return fail(QPX_ERR_ARG, "null equality pointer");
+  if (flags & ~QPX_TRACE_DEBUG_STEP) return fail(QPX_ERR_ARG, "unknown flags 0x%x", flags);
+  if (batch == 0) return QPX_OK;
+  auto launch = [&](auto tag) -> qpx_status {
+    using TS = decltype(tag);
+    auto a = make_args<TS>(batch, n, m_eq, m_ineq, workspace);
+    a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
+    a.G = (const TS*)G; a.h = (const TS*)h;
+    a.x = (TS*)x; a.s = (TS*)s; a.z = (TS*)z; a.y = (TS*)y; a.conv = converged; a.iters = iters;
+    a.tol = solver_tol; a.flags = flags; a.trace = trace;
+    const int lds = qpx::generic_lds_bytes(n, m_eq);
+    return dispatch_nt(n, m_ineq, [&](auto nt) {
+      hipLaunchKernelGGL((qpx::k_fused_solve<decltype(nt)::value, TS>), dim3((unsigned)batch),
+                         dim3(decltype(nt)::value), lds, (hipStream_t)stream, a, 0);
+      QPX_HIP(hipGetLastError());
+      return QPX_OK;
+    });
+  };
+  return dtype == QPX_F32 ? launch(float()) : launch(double());
 }
 
 }  // extern "C"

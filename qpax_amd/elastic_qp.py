@@ -124,3 +124,20 @@ def solve_qp_elastic_primal(Q, q, G, h, penalty, solver_tol: float = 1e-5, targe
     At = A if isinstance(A, torch.Tensor) else torch.as_tensor(A)
     bt_ = b if isinstance(b, torch.Tensor) else torch.as_tensor(b)
     return _SolveQPElasticPrimal.apply(*ts, penalty, At, bt_, solver_tol, target_kappa, mode, relax, exact_dG)
+
+
+def initialize_elastic(Q, q, G, h, penalty, A=None, b=None):
+    """qpax.elastic_qp.initialize_elastic (elastic_qp.py:27-47) -> (x, t, s1, s2, z1, z2), with Q
+    symmetrised as solve_qp_elastic hands it over (:186); y is appended with equality rows."""
+    bt, pen = _prep(Q, q, G, h, penalty, A, b)
+    lib = _lib.load()
+    n, me, mi = bt.n, bt.me, bt.mi
+    outs = [bt.out(n), bt.out(mi), bt.out(mi), bt.out(mi), bt.out(mi), bt.out(mi), bt.out(me)]
+    ws, nbytes = _workspace(bt)
+    st = lib.qpx_initialize_elastic(dtype_code(bt.dtype), bt.B, n, me, mi, *[ptr(bt[k]) for k in bt.names], ptr(pen),
+                                    *[ptr(o) for o in outs], ptr(ws), nbytes, stream_of(bt.device))
+    _lib.check(st, "qpx_initialize_elastic")
+    res = [bt.unflatten(outs[0], n)] + [bt.unflatten(v, mi) for v in outs[1:6]]
+    if A is not None:
+        res.append(bt.unflatten(outs[6], me))
+    return tuple(res)

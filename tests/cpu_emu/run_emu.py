@@ -12,6 +12,8 @@ non-zero exit status.  Case kinds (the npz's `kind`):
   me, mi; runs qpx_plan_create, qpx_plan_solve_relaxed and qpx_plan_diff with dl/dx = 2 (x - 1);
   optional `tune` = 1 runs qpx_plan_tune (every candidate launch shape) for the forward.
 * "uniform": batched arrays Q (B,n,n), ...; runs qpx_solve_qp_relaxed and qpx_diff_qp in `mode`.
+* "init", "init_elastic", "trace": qpx_initialize, qpx_initialize_elastic (with `penalty`) and
+  qpx_solve_qp_trace (with `tol`, `debug_step`) on batched arrays.
 
 The output npz holds every output array (flat, in the layout of the C ABI).
 """
@@ -160,11 +162,65 @@ def run_uniform(e: Emu, c) -> dict:
     return out
 
 
+def _batch_args(e: Emu, c, names):
+    dt = np.float32 if str(c["dtype"]) == "float32" else np.float64
+    arrs = [np.asarray(c[k], dt) for k in names]
+    return dt, (0 if dt == np.float32 else 1), arrs
+
+
+def run_init(e: Emu, c) -> dict:
+    """qpx_initialize on batched arrays Q, q, A, b, G, h."""
+    lib = e.lib
+    dt, code, (Q, q, A, b, G, h) = _batch_args(e, c, ("Q", "q", "A", "b", "G", "h"))
+    B, n, me, mi = Q.shape[0], Q.shape[1], A.shape[1], G.shape[1]
+    ops = [e.dev(v) for v in (Q, q, A, b, G, h)]
+    wsb = lib.qpx_workspace_size(B, n, me, mi)
+    ws = e.empty(wsb, np.uint8)
+    x, s, z, y = e.empty(B * n, dt), e.empty(B * mi, dt), e.empty(B * mi, dt), e.empty(B * me, dt)
+    e.check(lib.qpx_initialize(code, B, n, me, mi, *ops, x, s, z, y, ws, wsb, None), "initialize")
+    return {"x": e.host(x, B * n, dt), "s": e.host(s, B * mi, dt), "z": e.host(z, B * mi, dt),
+            "y": e.host(y, B * me, dt)}
+
+
+def run_init_elastic(e: Emu, c) -> dict:
+    """qpx_initialize_elastic on batched arrays Q, q, A, b, G, h and penalty (B,)."""
+    lib = e.lib
+    dt, code, (Q, q, A, b, G, h, pen) = _batch_args(e, c, ("Q", "q", "A", "b", "G", "h", "penalty"))
+    B, n, me, mi = Q.shape[0], Q.shape[1], A.shape[1], G.shape[1]
+    ops = [e.dev(v) for v in (Q, q, A, b, G, h, pen)]
+    wsb = lib.qpx_workspace_size(B, n, me, mi)
+    ws = e.empty(wsb, np.uint8)
+    outs = [e.empty(B * n, dt)] + [e.empty(B * mi, dt) for _ in range(5)] + [e.empty(B * me, dt)]
+    e.check(lib.qpx_initialize_elastic(code, B, n, me, mi, *ops, *outs, ws, wsb, None), "initialize_elastic")
+    sizes = [B * n] + [B * mi] * 5 + [B * me]
+    return {k: e.host(o, sz, dt) for k, o, sz in zip(("x", "t", "s1", "s2", "z1", "z2", "y"), outs, sizes)}
+
+
+def run_trace(e: Emu, c) -> dict:
+    """qpx_solve_qp_trace on batched arrays; `debug_step` selects QPX_TRACE_DEBUG_STEP."""
+    lib = e.lib
+    dt, code, (Q, q, A, b, G, h) = _batch_args(e, c, ("Q", "q", "A", "b", "G", "h"))
+    B, n, me, mi = Q.shape[0], Q.shape[1], A.shape[1], G.shape[1]
+    ops = [e.dev(v) for v in (Q, q, A, b, G, h)]
+    wsb = lib.qpx_workspace_size(B, n, me, mi)
+    ws = e.empty(wsb, np.uint8)
+    x, s, z, y = e.empty(B * n, dt), e.empty(B * mi, dt), e.empty(B * mi, dt), e.empty(B * me, dt)
+    conv, it = e.empty(B, np.int32), e.empty(B, np.int32)
+    tr = e.empty(B * 30 * 5, np.float64)
+    flags = 4 if int(c["debug_step"]) else 0
+    e.check(lib.qpx_solve_qp_trace(code, B, n, me, mi, *ops, float(c["tol"]), flags, x, s, z, y, conv, it, tr, ws,
+                                   wsb, None), "solve_qp_trace")
+    return {"x": e.host(x, B * n, dt), "s": e.host(s, B * mi, dt), "z": e.host(z, B * mi, dt),
+            "y": e.host(y, B * me, dt), "conv": e.host(conv, B, np.int32), "iters": e.host(it, B, np.int32),
+            "trace": e.host(tr, B * 150, np.float64).reshape(B, 30, 5)}
+
+
 def main(argv):
     case = np.load(argv[1], allow_pickle=False)
     e = Emu()
     kind = str(case["kind"])
-    out = {"ragged": run_ragged, "uniform": run_uniform}[kind](e, case)
+    out = {"ragged": run_ragged, "uniform": run_uniform, "init": run_init, "init_elastic": run_init_elastic,
+           "trace": run_trace}[kind](e, case)
     np.savez(argv[2], **out)
     print(f"emu: {kind} case done", flush=True)
 

@@ -164,3 +164,46 @@ def test_emulated_ragged_every_launch_shape(emu, tmp_path):
         probs.append((Q[0], q[0], A[0], b[0], G[0], h[0]))
     out = _run(emu, _ragged_case(probs, "float64", 1e-6, 1e-3, tune=1), tmp_path, "tune")
     _check_ragged("tune", probs, out, np.float64, 1e-6, 1e-3, range(len(probs)))
+
+
+def _golden():
+    d = np.load(os.path.join(HERE, "golden", "problem_data_pos.npz"))
+    iv = np.load(os.path.join(HERE, "golden", "initial_values_pos.npz"))
+    return d, iv
+
+
+def test_emulated_initialize_matches_reference_fixtures(emu, tmp_path):
+    """qpx_initialize on the expanded elastic problem and qpx_initialize_elastic on the elastic one
+    reproduce the reference's initial_values_pos.npz at its own 1e-6 (test/test_elastic_qp.py:79-95)."""
+    d, iv = _golden()
+    Q, q, G, h, pen = d["Q"][None], d["q"][None], d["G"][None], d["h"][None], float(d["penalty"])
+    Q2, q2, A2, b2, G2, h2 = O.expanded_elastic_problem(Q, q, G, h, pen)
+    out = _run(emu, dict(kind="init", dtype="float64", Q=Q2, q=q2, A=A2, b=b2, G=G2, h=h2), tmp_path, "init")
+    assert np.linalg.norm(out["x"] - np.concatenate([iv["x"], iv["t"]])) < 1e-6
+    assert np.linalg.norm(out["s"] - np.concatenate([iv["s1"], iv["s2"]])) < 1e-6
+    assert np.linalg.norm(out["z"] - np.concatenate([iv["z1"], iv["z2"]])) < 1e-6
+    A0, b0 = np.zeros((1, 0, 5)), np.zeros((1, 0))
+    oe = _run(emu, dict(kind="init_elastic", dtype="float64", Q=Q, q=q, A=A0, b=b0, G=G, h=h, penalty=np.array([pen])),
+              tmp_path, "init_el")
+    for k in ("x", "t", "s1", "s2", "z1", "z2"):
+        assert np.linalg.norm(oe[k] - iv[k]) < 1e-6, k
+
+
+@pytest.mark.parametrize("debug_step", [0, 1])
+def test_emulated_trace_matches_oracle(emu, tmp_path, debug_step):
+    """qpx_solve_qp_trace: per-iteration residual norms and step lengths against the oracle's
+    solve_qp_trace, for solve_qp's step rule and the reference debug loop's."""
+    data = list(P.random_feasible(4, 20, 5, 40, seed=2))
+    out = _run(emu, dict(kind="trace", dtype="float64", tol=1e-6, debug_step=debug_step,
+                         **dict(zip("Q q A b G h".split(), data))), tmp_path, f"trace{debug_step}")
+    (xo, so, zo, yo, co, io), tro = O.solve_qp_trace(*data, solver_tol=1e-6, debug_step=bool(debug_step))
+    np.testing.assert_array_equal(out["iters"], io)
+    np.testing.assert_array_equal(out["conv"], co)
+    tr = out["trace"]
+    assert np.array_equal(np.isnan(tr), np.isnan(tro))
+    # residual norms to 1e-6 relative above a floor of 1e-9 times the problem's first residual
+    # (the last iterations reach rounding-level values, where both sides are noise)
+    floor = 1e-9 * np.nanmax(tro[:, 0, :4], axis=1)[:, None, None]
+    err = np.abs(tr - tro) - (1e-6 * np.abs(tro) + floor)
+    assert np.nanmax(err[..., :4]) <= 0.0
+    assert np.nanmax(np.abs(tr[..., 4] - tro[..., 4])) < 1e-9

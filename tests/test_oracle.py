@@ -368,3 +368,33 @@ def test_conditioning_floor_separates_well_and_ill_conditioned_batches():
         fl = conditioning_floor(lambda *v: O.solve_qp(*v, solver_tol=tol), data, ref, ref[4] == 1)
         worst = max(fl.values())
         assert lo <= worst < hi, (n, fl)
+
+
+def test_trace_follows_the_solve_and_its_stopping_rule():
+    """solve_qp_trace: with solve_qp's step rule its iterate and flags are solve_qp's exactly; each
+    problem has `iters` trace rows, the last row's residual norms fall below the tolerance exactly
+    for the converged problems (the loop stops on the entry test, pdip.py:100), and every step
+    length lies in (0, 0.99].  With the debug loop's rule the step lengths are capped at
+    0.99 * 0.99 and the solution is the same QP's."""
+    data = list(P.random_feasible(24, 20, 5, 40, seed=2))
+    ref = O.solve_qp(*data, solver_tol=1e-6)
+    out, tr = O.solve_qp_trace(*data, solver_tol=1e-6)
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    iters = out[5]
+    for p in range(24):
+        k = iters[p]
+        assert np.isfinite(tr[p, :k]).all() and np.isnan(tr[p, k:]).all()
+        last = tr[p, k - 1, :4].max()
+        assert (last < 1e-6) == (out[4][p] == 1)
+        assert ((tr[p, :k, 4] > 0) & (tr[p, :k, 4] <= 0.99)).all()
+        assert (tr[p, :k - 1, :4].max(axis=1) >= 1e-6).all()
+    dbg, trd = O.solve_qp_trace(*data, solver_tol=1e-6, debug_step=True)
+    kd = dbg[5]
+    assert all((trd[p, :kd[p], 4] <= 0.99 * 0.99 + 1e-15).all() for p in range(24))
+    ok = (dbg[4] == 1) & (ref[4] == 1)
+    assert np.abs(dbg[0][ok] - ref[0][ok]).max() < 1e-4
+    # no equality rows: |r4| is 0, as the debug print's zeros(1)
+    nn = list(P.nnls(4, seed=1))
+    _, trn = O.solve_qp_trace(*nn, solver_tol=1e-8)
+    assert np.all(trn[:, 0, 3] == 0.0)
