@@ -87,6 +87,26 @@ QPX_API qpx_status qpx_solve_qp_relaxed(int32_t dtype, int32_t mode, int64_t bat
                                         int32_t* relax_converged, int32_t* relax_iters,
                                         void* workspace, size_t workspace_bytes, void* stream);
 
+/* qpx_solve_qp_relaxed (QPX_FUSED) without a workspace argument and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  Its scratch is a
+ * device array of qpx_noalloc_scratch_bytes() bytes (256 MiB) inside the library, one per GPU,
+ * held until the process ends; a call needs 4 KiB plus, per problem, 8 * (round64(m^2) + 640) bytes
+ * for sizes the wavefront kernels take (m = n rounded up to a multiple of 4, plus m_eq; 36 KB at
+ * n = 50, m_eq = 10, so 4096 such problems fit) and
+ * qpx_workspace_size(1, ...) - 4096 bytes for the others, and returns QPX_ERR_UNSUPPORTED beyond
+ * the array.  Calls on one GPU share the array: calls that overlap on one GPU (two streams, or two
+ * host threads) race.  Results are those of qpx_solve_qp_relaxed in QPX_FUSED mode, bit for bit. */
+QPX_API qpx_status qpx_solve_qp_relaxed_noalloc(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq,
+                                                int32_t m_ineq, const void* Q, const void* q, const void* A,
+                                                const void* b, const void* G, const void* h, double solver_tol,
+                                                double target_kappa, void* x, void* s, void* z, void* y,
+                                                int32_t* converged, int32_t* iters, void* xr, void* sr, void* zr,
+                                                void* yr, int32_t* relax_converged, int32_t* relax_iters,
+                                                void* stream);
+/* Bytes of the no-allocation entry point's scratch array. */
+QPX_API size_t qpx_noalloc_scratch_bytes(void);
+
 /* Backward of qpax.solve_qp_primal -- replaces solve_qp_primal_backward / diff_qp /
  * optnet_derivatives (qpax/diff_qp.py:8-46, :74-79): factor the KKT system at the relaxed point
  * (sr, zr), solve it with right-hand side (-dl_dx, 0, 0, 0) and form the six cotangents

@@ -36,6 +36,9 @@ SIGNATURES = {
     "qpx_diff_qp_elastic": (_i32, [_i32, _i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_vp] * 8 + [_i32] + [_vp] * 6
                             + [_vp, _sz, _vp]),
     "qpx_solve_qp_special": (_i32, [_i32, _i64, _i32, _i32] + [_vp] * 4 + [_vp] * 4 + [_vp, _sz, _vp]),
+    "qpx_solve_qp_relaxed_noalloc": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_f64, _f64] + [_vp] * 12
+                                     + [_vp]),
+    "qpx_noalloc_scratch_bytes": (_sz, []),
     "qpx_initialize": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_vp] * 4 + [_vp, _sz, _vp]),
     "qpx_initialize_elastic": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 7 + [_vp] * 7 + [_vp, _sz, _vp]),
     "qpx_solve_qp_trace": (_i32, [_i32, _i64, _i32, _i32, _i32] + [_vp] * 6 + [_f64, _i32] + [_vp] * 7

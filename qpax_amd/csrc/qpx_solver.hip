@@ -690,8 +690,9 @@ qpx_status forward_impl(int mode, int64_t B, int n, int me, int mi, const void* 
                         const void* A, const void* b, const void* G, const void* h, double tol,
                         double kappa, bool relax, void* x, void* s, void* z, void* y, int* conv,
                         int* iters, void* xr, void* sr, void* zr, void* yr, int* rconv,
-                        int* riters, void* ws, void* stream) {
+                        int* riters, void* ws, void* stream, int64_t stride = 0) {
   auto a = make_args<TS>(B, n, me, mi, ws);
+  if (stride > 0) a.stride = stride;
   a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
   a.G = (const TS*)G; a.h = (const TS*)h;
   a.x = (TS*)x; a.s = (TS*)s; a.z = (TS*)z; a.y = (TS*)y; a.conv = conv; a.iters = iters;
@@ -810,11 +811,84 @@ bool any_null(std::initializer_list<const void*> ps) {
   return false;
 }
 
+// ------------------------------ the entry point that allocates nothing -------------------------
+// Scratch of qpx_solve_qp_relaxed_noalloc: a file-scope device array, one copy on every GPU the
+// library runs on, held until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the per-device address from
+// hipGetSymbolAddress, taken once per device on the entry point's first call there.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+qpx_status scratch_of_current_device(void** out) {
+  int dev = -1;
+  QPX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(QPX_ERR_UNSUPPORTED, "device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(QPX_ERR_HIP, "hipGetSymbolAddress of the scratch array failed on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return QPX_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 QPX_API int32_t qpx_version(void) { return 100; }
+
+QPX_API size_t qpx_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+QPX_API qpx_status qpx_solve_qp_relaxed_noalloc(int32_t dtype, int64_t batch, int32_t n, int32_t m_eq,
+                                                int32_t m_ineq, const void* Q, const void* q, const void* A,
+                                                const void* b, const void* G, const void* h, double solver_tol,
+                                                double target_kappa, void* x, void* s, void* z, void* y,
+                                                int32_t* converged, int32_t* iters, void* xr, void* sr, void* zr,
+                                                void* yr, int32_t* relax_converged, int32_t* relax_iters,
+                                                void* stream) {
+  qpx::host::clear_error();
+  if (dtype != QPX_F32 && dtype != QPX_F64) return fail(QPX_ERR_ARG, "dtype must be QPX_F32 or QPX_F64 (got %d)", dtype);
+  if (batch < 0 || n < 1 || m_eq < 0 || m_ineq < 1)
+    return fail(QPX_ERR_ARG, "invalid sizes batch=%lld n=%d m_eq=%d m_ineq=%d (m_ineq >= 1)", (long long)batch, n,
+                m_eq, m_ineq);
+  if (batch > 0x7fffffffLL) return fail(QPX_ERR_UNSUPPORTED, "batch %lld exceeds 2^31-1", (long long)batch);
+  if (batch > 0 && any_null({Q, q, G, h, x, s, z, converged, iters, xr, sr, zr, relax_converged, relax_iters}))
+    return fail(QPX_ERR_ARG, "null pointer argument");
+  if (batch > 0 && m_eq > 0 && any_null({A, b, y, yr})) return fail(QPX_ERR_ARG, "null equality pointer");
+  // per-problem words: the wavefront kernels keep only the m x m constant matrix and the split
+  // launch's hand-over state; the other kernels their whole workspace view
+  qpx::host::WaveVariant wv;
+  const int ts = dtype == QPX_F32 ? 4 : 8;
+  const int64_t stride = qpx::host::wave_variant(n, m_eq, m_ineq, ts, &wv)
+                             ? qpx::round_up(qpx::wave::split_words(n, m_eq), 64)
+                             : stride_words(n, m_eq, m_ineq);
+  const size_t need = kCtlBytes + (size_t)batch * (size_t)stride * sizeof(double);
+  if (need > kNoAllocScratchBytes)
+    return fail(QPX_ERR_UNSUPPORTED,
+                "batch %lld of n=%d m_eq=%d m_ineq=%d needs %zu bytes of scratch, beyond the %zu of the no-allocation "
+                "entry point (use qpx_solve_qp_relaxed with a workspace)",
+                (long long)batch, n, m_eq, m_ineq, need, kNoAllocScratchBytes);
+  if (batch == 0) return QPX_OK;
+  void* ws = nullptr;
+  qpx_status st = scratch_of_current_device(&ws);
+  if (st != QPX_OK) return st;
+  if (dtype == QPX_F32)
+    return forward_impl<float>(QPX_FUSED, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, solver_tol, target_kappa, true, x,
+                               s, z, y, converged, iters, xr, sr, zr, yr, relax_converged, relax_iters, ws, stream,
+                               stride);
+  return forward_impl<double>(QPX_FUSED, batch, n, m_eq, m_ineq, Q, q, A, b, G, h, solver_tol, target_kappa, true, x,
+                              s, z, y, converged, iters, xr, sr, zr, yr, relax_converged, relax_iters, ws, stream,
+                              stride);
+}
 
 QPX_API const char* qpx_last_error(void) { return qpx::host::last_error(); }
 

@@ -25,7 +25,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 
 // the kernels' register-pinning asm and their dynamic-LDS declaration, in host form
 #define QPX_PIN_V(x) asm volatile("" : "+r"(x))
@@ -90,6 +90,12 @@ hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipStreamWaitEvent(hipStream_t st, hipEvent_t e, unsigned flags);
 hipError_t hipGetLastError();
+#define HIP_SYMBOL(X) X
+template <typename T>
+inline hipError_t hipGetSymbolAddress(void** p, const T& symbol) {
+  *p = (void*)&symbol;
+  return hipSuccess;
+}
 const char* hipGetErrorString(hipError_t e);
 
 namespace emu {
