@@ -64,7 +64,7 @@ def profiled_traffic(kernel):
 
 
 def make_inputs(torch, dev):
-    from qpax_amd import problems as P
+    from tests import problems as P
 
     data = [np.asarray(v, np.float32) for v in P.random_feasible(BATCH, N, ME, MI, seed=SEED)]
     return data, [torch.tensor(v, device=dev, requires_grad=True) for v in data]
@@ -104,7 +104,7 @@ def kernel_times(torch, qpax_amd, ts, reps):
     for r in range(reps + 1):
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record(st)
-        out = solve_qp_relaxed_batch(bt, TOL, KAPPA, "fused")
+        out = solve_qp_relaxed_batch(bt, TOL, KAPPA, "fused", check=False)
         e1.record(st)
         g = 2.0 * (out[0] - 1.0)
         e1b = torch.cuda.Event(enable_timing=True)
@@ -180,7 +180,7 @@ def uneven_bench(torch, args):
     """Uneven batching: 16384 QPs with n ~ U{8..128}, m_ineq ~ U{n..3n}, m_eq ~ U{0..n/4} (seed 4),
     float32 fwd (solve + relax) + bwd through one autotuned ragged plan, captured in a CUDA graph
     and replayed; one step = one replay."""
-    from qpax_amd import problems as P
+    from tests import problems as P
     from qpax_amd.ragged import RaggedPlan
 
     B = args.uneven_batch
@@ -261,7 +261,7 @@ def elastic_bench(torch, args):
     h -= 3|N(0,1)|, penalty 14, float32 fwd (solve + relax) + bwd through solve_qp_elastic_primal
     with loss ||x - 1||^2; one step = forward + backward of the whole batch."""
     import qpax_amd
-    from qpax_amd import problems as P
+    from tests import problems as P
 
     B = args.elastic_batch
     Q, q, A, b, G, h, shifted = P.elastic_infeasible(B, 256, 512, frac=0.2, seed=5, me=32)
@@ -310,6 +310,48 @@ def elastic_bench(torch, args):
     print(json.dumps(res), flush=True)
 
 
+def nnls_bench(torch, args):
+    """The NNLS batch example's workload (qpax's README and examples/batch_qp.py:14-33): 10000
+    nonnegative least-squares QPs, n=5, G=-I, h=0, no equality rows, forward solve_qp only, in
+    float64 at solver_tol 1e-10 and in float32 at 1e-5.  One step = one solve of the whole batch;
+    `value` is the float64 rate, the float32 one is reported beside it."""
+    import qpax_amd
+    from tests import problems as P
+
+    B = 10000
+    res = {"metric": "forward QPs/s, 10000 NNLS QPs n=5 (G=-I: box rows on the diagonal path)", "unit": "QPs/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "10000 NNLS QPs, F in R^{10x5}, g in R^10 ~ N(0,1), Q=F^T F, q=-F^T g, G=-I5, h=0, "
+                                  "seed 1; solve_qp forward", "batch": B}}
+    data = P.nnls(B, seed=1)
+    for dt, tol, key in ((np.float64, 1e-10, "f64"), (np.float32, 1e-5, "f32")):
+        ts = [torch.tensor(np.asarray(v, dt), device="cuda") for v in data]
+        out = None
+
+        def step():
+            nonlocal out
+            out = qpax_amd.solve_qp(*ts, solver_tol=tol)
+
+        ms = 1e3 * time_steps(torch, step, args.steps, args.warmup)
+        x = out[0].cpu().numpy().astype(np.float64)
+        conv = out[4].cpu().numpy()
+        entry = {"qps": round(B / (ms * 1e-3), 1), "ms_per_step": round(ms, 4), "solver_tol": tol,
+                 "converged_frac": float((conv == 1).mean()), "iters_mean": float(out[5].float().mean())}
+        if not args.no_cpu:
+            from oracle import qp_oracle as O
+
+            d64 = [np.asarray(np.asarray(v, dt), np.float64) for v in data]
+            xo, so, zo, yo, co, io = O.solve_qp(*d64, solver_tol=tol)
+            ok = co == 1
+            err = np.linalg.norm(x - xo, axis=1) / np.maximum(np.linalg.norm(xo, axis=1), 1e-12)
+            entry["x_rel_err_vs_f64_oracle_max"] = float(err[ok].max())
+        res[key] = entry
+    res["value"] = res["f64"]["qps"]
+    res["ms_per_step"] = res["f64"]["ms_per_step"]
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,7 +362,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only run a few fused steps (for rocprofv3)")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a plain device copy (counter calibration)")
-    ap.add_argument("--workload", choices=("c3", "uneven", "elastic"), default="c3",
+    ap.add_argument("--workload", choices=("c3", "uneven", "elastic", "nnls"), default="c3",
                     help="c3: the headline n=50 B=4096 workload; uneven: 16384 mixed-size QPs (ragged plan); "
                          "elastic: 1024 elastic QPs n=256 m_eq=32 m_ineq=512")
     ap.add_argument("--elastic-batch", type=int, default=1024)
@@ -335,7 +377,7 @@ def main():
     cpu = None
     if args.workload == "c3" and not args.no_cpu and not args.profile and not args.copy_calib:
         # the CPU baseline first, before this process initialises the GPU (its workers are forks)
-        from qpax_amd import problems as P
+        from tests import problems as P
 
         data = [np.asarray(v, np.float32) for v in P.random_feasible(BATCH, N, ME, MI, seed=SEED)]
         cpu = cpu_baseline(data, min(args.cpu_sample, BATCH))
@@ -360,6 +402,9 @@ def main():
         return
     if args.workload == "elastic":
         elastic_bench(torch, args)
+        return
+    if args.workload == "nnls":
+        nnls_bench(torch, args)
         return
 
     data, ts = make_inputs(torch, dev)

@@ -171,6 +171,41 @@ __device__ __forceinline__ double grow_dot2(const TS* __restrict__ Gs, int n, in
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
+// Box rows -- the +-I blocks of box-constrained and nonnegative QPs (G = -I in the NNLS batch
+// example, G = [G_dense; I; -I] in the reference's QPTEST).  load() finds md, one past the last
+// row of G that is not +-e_j or zero; rows md..mi-1 are each +-e_j (or zero, e.g. a row with
+// infinite h), and their code sigma * (j + 1) (0 for a zero row) replaces the row's first entry in
+// LDS.  The dense streams stop at md, and those rows enter as a diagonal (G^T D G), a gather
+// (G^T v) and one entry (G x).
+template <typename TS>
+__device__ __forceinline__ void box_code(const TS* __restrict__ Gs, int n, int k, int& j, double& sg) {
+  const double cv = (double)Gs[k * n];
+  sg = cv > 0.0 ? 1.0 : (cv < 0.0 ? -1.0 : 0.0);
+  j = (int)fabs(cv) - 1;
+}
+// sum of sigma_k^P v[k] over the box rows k in [md, mi) whose column j_k is col (P = 1: G^T v;
+// P = 2: the diagonal of G^T diag(v) G)
+template <int P, typename TS>
+__device__ __forceinline__ double box_gather(const TS* __restrict__ Gs, int n, int md, int mi, int col,
+                                             const double* __restrict__ v) {
+  double a = 0.0;
+  for (int k = md; k < mi; ++k) {
+    int j;
+    double sg;
+    box_code(Gs, n, k, j, sg);
+    a += j == col ? (P == 2 ? sg * sg : sg) * v[k] : 0.0;
+  }
+  return a;
+}
+// (G x)_k of box row k, x broadcast from LDS
+template <typename TS>
+__device__ __forceinline__ double box_row(const TS* __restrict__ Gs, int n, int k, const double* __restrict__ x) {
+  int j;
+  double sg;
+  box_code(Gs, n, k, j, sg);
+  return sg * x[j < 0 ? 0 : j];
+}
+
 // 1/d to full double precision: hardware reciprocal plus two Newton steps.
 __device__ __forceinline__ double recip(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -196,6 +231,7 @@ __device__ __forceinline__ double recip_c(double d) {
 
 template <typename TS, int NMAX, int MIC>
 struct St {
+  int md;         // rows of G streamed densely; rows md..mi-1 are box rows (see box_code)
   double qa;      // q (rows < n), -b (rows np..m-1), 0 otherwise
   double h[MIC];
   bool hinf[MIC];
@@ -394,6 +430,39 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
     for (int e = gnv * VW + l; e < gtot; e += 64) Gs[e] = G[e];
     zero_inf_rows();
   }
+  // box rows: md = one past the last row that is not +-e_j or zero; rows md..mi-1 get their codes
+  WSYNC();
+  {
+    double last = 0.0, code[MIC];
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      code[cc] = 0.0;
+      if (k < mi) {
+        // a row with a second nonzero is dense: general rows end the scan after two entries
+        int nnz = 0, jj = 0;
+        double v1 = 0.0;
+        for (int j = 0; j < n && nnz < 2; ++j) {
+          const double g = (double)Gs[k * n + j];
+          if (g != 0.0) {
+            ++nnz;
+            jj = j;
+            v1 = g;
+          }
+        }
+        const bool box = nnz == 0 || (nnz == 1 && (v1 == 1.0 || v1 == -1.0));
+        code[cc] = nnz == 0 ? 0.0 : (v1 > 0.0 ? 1.0 : -1.0) * (jj + 1);
+        if (!box) last = (double)(k + 1);
+      }
+    }
+    st.md = (int)wmax(last);
+    WSYNC();
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      if (k >= st.md && k < mi) Gs[k * n] = (TS)code[cc];
+    }
+  }
   __threadfence_block();
   if constexpr (FAST) tile_load(c, st.Tk);
   st.qa = l < n ? (double)q[l] : ((l >= np && l < m) ? -(double)b[l - np] : 0.0);
@@ -405,7 +474,7 @@ __device__ __forceinline__ void load(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, M
 // (NaN-propagating).
 template <typename TS, int NMAX, int MIC>
 __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double kappa) {
-  const int n = opaque_s(c.n), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
+  const int n = opaque_s(c.n), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l), md = opaque_s(st.md);
   c.col[l] = st.xa;
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
@@ -434,14 +503,14 @@ __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     for (int u = 1; u < 16; ++u) a[0] += a[u];
     if (l < m) acc += a[0];
   }
-  if (l < n) acc += gcol_dot(c.Gs, n, mi, l, c.vb);
+  if (l < n) acc += gcol_dot(c.Gs, n, md, l, c.vb) + (md < mi ? box_gather<1>(c.Gs, n, md, mi, l, c.vb) : 0.0);
   st.ra = l < m ? acc : 0.0;
   double nrm = fabs(st.ra);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if (k < mi) {
-      const double g = grow_dot(c.Gs, n, k, c.col);
+      const double g = k < md ? grow_dot(c.Gs, n, k, c.col) : box_row(c.Gs, n, k, c.col);
       st.r3[cc] = g + st.s[cc] - st.h[cc];
       st.r2[cc] = st.s[cc] * st.z[cc] - kappa;
       nrm = nmax(nrm, nmax(fabs(st.r2[cc]), fabs(st.r3[cc])));
@@ -455,6 +524,38 @@ __device__ __forceinline__ double residuals(const Ctx<TS, NMAX, MIC>& c, St<TS, 
 }
 
 __host__ __device__ constexpr int tidx(int I, int J) { return I * (I + 1) / 2 + J; }
+
+// The box rows' part of a factorisation pass (no-op without box rows): D_k on the diagonal of the
+// tiles (diagonal tile J holds (16J + c, 16J + c) in register c / 4 of lane 16 (c % 4) + c, the
+// same in the lower- and the upper-triangle layouts), and with RES their entries of G x (into vb,
+// after the D values there are consumed) and this lane's column of their G^T z (returned).
+template <typename TS, int NMAX, int MIC, int NTILE>
+__device__ __forceinline__ double box_rows_into(const Ctx<TS, NMAX, MIC>& c, const St<TS, NMAX, MIC>& st,
+                                                d4 (&T)[NTILE], int md, bool res) {
+  constexpr int NTI = NMAX / 16;
+  const int n = c.n, mi = c.mi, l = c.l, rq = l >> 4, cq = l & 15;
+  if (md >= mi) return 0.0;
+  const double dsum = box_gather<2>(c.Gs, n, md, mi, l, c.vb);
+  const bool holds = rq == (cq & 3);
+#pragma unroll
+  for (int J = 0; J < NTI; ++J) {
+    const double v = __shfl(dsum, 16 * J + cq, 64);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) T[tidx(J, J)][r] += (holds && (cq >> 2) == r) ? v : 0.0;
+  }
+  double zb = 0.0;
+  if (res) {
+    zb = l < n ? box_gather<1>(c.Gs, n, md, mi, l, c.vb + mi) : 0.0;
+    WSYNC();
+#pragma unroll
+    for (int cc = 0; cc < MIC; ++cc) {
+      const int k = 64 * cc + l;
+      if (k >= md && k < mi) c.vb[k] = box_row(c.Gs, n, k, c.col);
+    }
+    WSYNC();
+  }
+  return zb;
+}
 
 // Factorisation (qpax/pdip.py:38-46): D = z/s (or 1 for the initialisation), K = KQ + G^T D G,
 // K = L D L^T with L unit lower (rows packed into c.Lp) and 1/D in st.rdpiv.  A pivot that is not
@@ -472,6 +573,7 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
   constexpr int NTI = NMAX / 16;
   constexpr int NTILE = NTI * (NTI + 1) / 2;
   const int n = opaque_s(c.n), np = opaque_s(c.np), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
+  const int md = opaque_s(st.md);
   const int rq = l >> 4, cq = l & 15;
   double* vz = c.vb + mi;  // z, for G^T z
   double* kx = c.col + 64; // K's constant part times [x; y], then plus G^T z
@@ -554,18 +656,18 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
     }
     auto fetch = [&](int k0, double (&g)[NTI], double& dk, double& zk) {
       const int k = k0 + rq;
-      const int kc = k < mi ? k : mi - 1;
-      dk = k < mi ? c.vb[kc] : 0.0;
-      zk = (RES && k < mi) ? vz[kc] : 0.0;
+      const int kc = k < md ? k : (md > 0 ? md - 1 : 0);
+      dk = k < md ? c.vb[kc] : 0.0;
+      zk = (RES && k < md) ? vz[kc] : 0.0;
       const TS* grow = c.Gs + kc * n;
 #pragma unroll
       for (int I = 0; I < NTI; ++I) g[I] = (double)grow[off[I]] * msk[I];
     };
     double gc[NTI], dc, zc;
     fetch(0, gc, dc, zc);
-    for (int k0 = 0; k0 < mi; k0 += 4) {
+    for (int k0 = 0; k0 < md; k0 += 4) {
       double gn[NTI], dn, zn;
-      fetch(k0 + 4 < mi ? k0 + 4 : k0, gn, dn, zn);
+      fetch(k0 + 4 < md ? k0 + 4 : k0, gn, dn, zn);
 #pragma unroll
       for (int I = 0; I < NTI; ++I) {
         const double a = gc[I] * dc;
@@ -582,7 +684,7 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
         }
         gx = row16_sum(gx);
         const int k = k0 + rq;
-        if (cq == 0 && k < mi) c.vb[k] = gx;
+        if (cq == 0 && k < md) c.vb[k] = gx;
       }
 #pragma unroll
       for (int I = 0; I < NTI; ++I) gc[I] = gn[I];
@@ -591,6 +693,7 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
     }
   }
   WSYNC();
+  const double zbox = box_rows_into(c, st, T, md, RES);
   double nrm = 0.0;
   if (RES) {
     // columns of the off-diagonal tiles and of G^T z: sum over the lane groups, add to kx
@@ -606,7 +709,7 @@ __device__ __forceinline__ double factor(const Ctx<TS, NMAX, MIC>& c, St<TS, NMA
       for (int J = 0; J < NTI; ++J) kx[16 * J + cq] += pc[J];
     }
     WSYNC();
-    st.ra = l < m ? kx[l] + st.qa : 0.0;
+    st.ra = l < m ? kx[l] + st.qa + zbox : 0.0;
     nrm = fabs(st.ra);
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) {
@@ -771,6 +874,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
   constexpr int NTI = NMAX / 16;
   constexpr int NTILE = NTI * (NTI + 1) / 2;
   const int n = opaque_s(c.n), np = opaque_s(c.np), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
+  const int md = opaque_s(st.md);
   const int rq = l >> 4, cq = l & 15;
   double* vz = c.vb + mi;  // z, for G^T z
   double* kx = c.col + 64; // K's constant part times [x; y], then plus G^T z
@@ -846,7 +950,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     const double msks = cs < n ? 1.0 : 0.0;
     // raw loads of a step's G row entries (input precision), D and z; converted later
     auto load_raw = [&](int k0, TS (&g)[NTI], TS& gs, double& dk, double& zk, bool with_strip) {
-      const int kc = k0 + rq < mi ? k0 + rq : mi - 1;
+      const int kc = k0 + rq < md ? k0 + rq : (md > 0 ? md - 1 : 0);
       dk = c.vb[kc];
       zk = RES ? vz[kc] : 0.0;
       const TS* grow = c.Gs + kc * n;
@@ -863,7 +967,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
         TS g0[NTI], gs0;
         double d0, z0;
         load_raw(0, g0, gs0, d0, z0, S);
-        const bool live = rq < mi;
+        const bool live = rq < md;
 #pragma unroll
         for (int I = 0; I < NTI; ++I) {
           gc[I] = (double)g0[I] * msk[I];
@@ -872,12 +976,12 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
         if (S) gsc = (double)gs0 * msks;
         zc = live ? z0 : 0.0;
       }
-      for (int k0 = 0; k0 < mi; k0 += 4) {
-        const int kn = k0 + 4 < mi ? k0 + 4 : k0;
+      for (int k0 = 0; k0 < md; k0 += 4) {
+        const int kn = k0 + 4 < md ? k0 + 4 : k0;
         TS gr[NTI], gsr;
         double dr, zr;
         load_raw(kn, gr, gsr, dr, zr, S);
-        const bool live_n = kn + rq < mi;
+        const bool live_n = kn + rq < md;
         double gx = 0.0;
         // the step's vector work, issued after its MFMAs (float64 MFMAs occupy the SIMD's vector
         // datapath, so nothing is hidden by interleaving them): this step's G x row sums and G^T z
@@ -894,7 +998,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
             if (g == 4) {
               gx += dpp<0x140>(gx);
               const int k = k0 + rq;
-              if (cq == 0 && k < mi) c.vb[k] = gx;
+              if (cq == 0 && k < md) c.vb[k] = gx;
             }
             if (g == 5) {
 #pragma unroll
@@ -961,6 +1065,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
     }
   }
   WSYNC();
+  const double zbox = box_rows_into(c, st, T, md, RES);
   double nrm = 0.0;
   if (RES) {
 #pragma unroll
@@ -975,7 +1080,7 @@ __device__ __forceinline__ double factor_up(const Ctx<TS, NMAX, MIC>& c, St<TS, 
       for (int I = 0; I < NTI; ++I) kx[16 * I + cq] += pc[I];
     }
     WSYNC();
-    st.ra = l < m ? kx[l] + st.qa : 0.0;
+    st.ra = l < m ? kx[l] + st.qa + zbox : 0.0;
     nrm = fabs(st.ra);
 #pragma unroll
     for (int cc = 0; cc < MIC; ++cc) {
@@ -1125,14 +1230,14 @@ template <bool FAST, typename TS, int NMAX, int MIC>
 __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, MIC>& st, double pa,
                                       const double (&a2)[MIC], const double (&a3)[MIC], double sg,
                                       double (&ods)[MIC], double (&odz)[MIC]) {
-  const int n = opaque_s(c.n), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l);
+  const int n = opaque_s(c.n), mi = opaque_s(c.mi), m = opaque_s(c.m), l = opaque_v(c.l), md = opaque_s(st.md);
 #pragma unroll
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if (k < mi) c.vb[k] = st.D[cc] * (sg * a3[cc] - (sg * a2[cc]) / st.z[cc]);
   }
   WSYNC();
-  if (l < n) pa += gcol_dot(c.Gs, n, mi, l, c.vb);
+  if (l < n) pa += gcol_dot(c.Gs, n, md, l, c.vb) + (md < mi ? box_gather<1>(c.Gs, n, md, mi, l, c.vb) : 0.0);
   if (l >= m) pa = 0.0;
   // forward: unit-lower L y = p; lane l reads row lr = min(l, m-1) of L.  The factor entries
   // load 16 at a time ahead of the dependency chain.  Reads past the end of a row run into the
@@ -1238,9 +1343,14 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
   for (int cc = 0; cc < MIC; ++cc) {
     const int k = 64 * cc + l;
     if constexpr (FAST) {
-      // every lane forms a row product (rows past m_ineq clamped to the last) and selects
-      const int kc = k < mi ? k : mi - 1;
-      const double g = (n & 1) == 0 ? grow_dot2(c.Gs, n, kc, c.col) : grow_dot(c.Gs, n, kc, c.col);
+      // every lane forms a row product (rows past the dense ones clamped to the last of them) and
+      // selects; box rows take their single entry instead
+      const int kc = k < md ? k : (md > 0 ? md - 1 : 0);
+      double g = (n & 1) == 0 ? grow_dot2(c.Gs, n, kc, c.col) : grow_dot(c.Gs, n, kc, c.col);
+      if (md < mi) {
+        const double gb = box_row(c.Gs, n, k < mi ? k : mi - 1, c.col);
+        g = k < md ? g : gb;
+      }
       const double v3 = sg * a3[cc], v2 = sg * a2[cc];
       const double dsk = v3 - g;
       const double dzk = (v2 - st.z[cc] * dsk) / st.s[cc];
@@ -1248,7 +1358,7 @@ __device__ __forceinline__ void solve(const Ctx<TS, NMAX, MIC>& c, St<TS, NMAX, 
       odz[cc] = k < mi ? dzk : 0.0;
     } else {
       if (k < mi) {
-        const double g = grow_dot(c.Gs, n, k, c.col);
+        const double g = k < md ? grow_dot(c.Gs, n, k, c.col) : box_row(c.Gs, n, k, c.col);
         const double v3 = sg * a3[cc], v2 = sg * a2[cc];
         const double dsk = v3 - g;
         ods[cc] = dsk;

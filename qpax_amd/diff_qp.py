@@ -54,7 +54,10 @@ class _SolveQPPrimal(torch.autograd.Function):
             return _special(bt)[0]
         need = any(ctx.needs_input_grad[:6])
         if need:
-            out = solve_qp_relaxed_batch(bt, solver_tol, target_kappa, mode)
+            # no read-back of the relaxation flags here (it would hold the host until the forward
+            # ends): a relaxation that did not start leaves NaN at its relaxed point, so the
+            # backward returns NaN cotangents for that problem
+            out = solve_qp_relaxed_batch(bt, solver_tol, target_kappa, mode, check=False)
             x, relaxed = out[0], out[6:10]
             ctx.save_for_backward(*[bt[k] for k in bt.names], *relaxed)
             ctx.meta = (bt.batch_shape, bt.n, bt.me, bt.mi, mode, [tuple(t.shape) for t in (Q, q, A, b, G, h)],

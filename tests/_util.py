@@ -4,12 +4,18 @@ import numpy as np
 
 
 def rel_err(a, b):
-    """Per-problem relative 2-norm error ||a-b|| / max(||b||, 1e-12) over all but the batch axis."""
+    """Per-problem relative 2-norm error ||a-b|| / max(||b||, 1e-12) over all but the batch axis.
+    Entries that are infinite in both (s on rows with infinite h) count as equal; an infinity in
+    one only makes the error infinite."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    ax = tuple(range(1, a.ndim))
-    return np.linalg.norm((a - b).reshape(a.shape[0], -1), axis=1) / np.maximum(
+    both = np.isinf(a) & np.isinf(b) & (np.sign(a) == np.sign(b))
+    one = (np.isinf(a) | np.isinf(b)) & ~both
+    a = np.where(both | one, 0.0, a)
+    b = np.where(both | one, 0.0, b)
+    err = np.linalg.norm((a - b).reshape(a.shape[0], -1), axis=1) / np.maximum(
         np.linalg.norm(b.reshape(b.shape[0], -1), axis=1), 1e-12)
+    return np.where(one.reshape(one.shape[0], -1).any(axis=1), np.inf, err)
 
 
 def to_np(t):

@@ -1,4 +1,4 @@
-"""Synthetic QP batches for the benchmark and the parity tests.
+"""Synthetic QP batches for the parity tests and the benchmark (test data, not part of the library).
 
 Every generator draws from ``numpy.random.default_rng(seed)`` in float64 and returns numpy
 arrays; callers cast to float32 for single-precision runs.  Batched draws (one array per operand,
@@ -116,3 +116,28 @@ def elastic_infeasible(B, n, mi, frac=0.2, seed=5, me=0):
     if me:
         return Q, q, A, b, G, h, shifted
     return Q, q, G, h, shifted
+
+
+def box_qp(B, n, me, dense, seed=0, inf_frac=0.0):
+    """Random QPs with bounds lb <= x <= ub below `dense` general rows, laid out as the reference's
+    test_QPTEST does (test/test_constraint_sizes.py:135-163): G = [G_dense; I; -I],
+    h = [G_dense x0 + |N(0,1)| + 0.1; ub; -lb] with x0 ~ U(-0.5, 0.5) strictly inside,
+    lb = -1 - U(0,1), ub = 1 + U(0,1), and a fraction inf_frac of the upper bounds set to +inf
+    (rows the solver drops, pdip.py:166-178).  Q = M M^T + 1e-3 I, q, A ~ N(0,1), b = A x0."""
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((B, n, n))
+    Q = M @ np.swapaxes(M, -1, -2) + 1e-3 * np.eye(n)
+    Q = 0.5 * (Q + np.swapaxes(Q, -1, -2))
+    q = rng.standard_normal((B, n))
+    A = rng.standard_normal((B, me, n))
+    x0 = rng.uniform(-0.5, 0.5, (B, n))
+    b = np.einsum("bij,bj->bi", A, x0)
+    Gd = rng.standard_normal((B, dense, n))
+    hd = np.einsum("bij,bj->bi", Gd, x0) + np.abs(rng.standard_normal((B, dense))) + 0.1
+    lb = -1.0 - rng.uniform(0.0, 1.0, (B, n))
+    ub = 1.0 + rng.uniform(0.0, 1.0, (B, n))
+    ub = np.where(rng.random((B, n)) < inf_frac, np.inf, ub)
+    eye = np.broadcast_to(np.eye(n), (B, n, n))
+    G = np.concatenate([Gd, eye, -eye], axis=1)
+    h = np.concatenate([hd, ub, -lb], axis=1)
+    return Q, q, A, b, G, h

@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 from oracle import qp_oracle as O
-from qpax_amd import problems as P
+from tests import problems as P
 from tests._util import rel_err
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -207,3 +207,42 @@ def test_emulated_trace_matches_oracle(emu, tmp_path, debug_step):
     err = np.abs(tr - tro) - (1e-6 * np.abs(tro) + floor)
     assert np.nanmax(err[..., :4]) <= 0.0
     assert np.nanmax(np.abs(tr[..., 4] - tro[..., 4])) < 1e-9
+
+
+# box rows (+-e_j, the G = -I of NNLS and the [G_dense; I; -I] of QPTEST): the dense streams stop
+# at the last general row and the box rows enter as a diagonal, a gather and single entries
+BOX_CASES = [
+    ("nnls-f64", lambda: P.nnls(3, seed=1), np.float64, 1e-8),
+    ("box20-me3-f32", lambda: P.box_qp(3, 20, 3, 10, seed=5), np.float32, 1e-6),
+    ("box40-dense0-f64", lambda: P.box_qp(3, 40, 4, 0, seed=6), np.float64, 1e-6),
+    ("box50-me8-f64", lambda: P.box_qp(3, 50, 8, 20, seed=7), np.float64, 1e-6),
+]
+
+
+@pytest.mark.parametrize("tag,gen,dt,tol", BOX_CASES, ids=[c[0] for c in BOX_CASES])
+def test_emulated_box_rows(emu, tmp_path, tag, gen, dt, tol):
+    data = [np.asarray(v, dt).astype(np.float64) for v in gen()]
+    B, n, me, mi = data[0].shape[0], data[0].shape[1], data[2].shape[1], data[4].shape[1]
+    case = dict(kind="uniform", dtype=np.dtype(dt).name, mode="fused", tol=tol, kappa=1e-3,
+                **dict(zip("Q q A b G h".split(), data)))
+    out = _run(emu, case, tmp_path, tag, EMU_CUS=1)
+    _check(tag, data, out, dt, tol, 1e-3, B, (n, me, mi))
+
+
+def test_emulated_box_rows_with_infinite_bounds(emu, tmp_path):
+    """Upper bounds of +inf: those rows are zeroed like every row with infinite h, and as zero rows
+    they sit among the box rows; the forward matches the oracle (the relaxation of such rows is
+    NaN in qpax, so only the solve is compared)."""
+    data = list(P.box_qp(3, 24, 2, 6, seed=8, inf_frac=0.3))
+    assert np.isinf(data[5]).any()
+    B, n, me, mi = 3, 24, 2, data[4].shape[1]
+    case = dict(kind="uniform", dtype="float64", mode="fused", tol=1e-6, kappa=1e-3,
+                **dict(zip("Q q A b G h".split(), data)))
+    out = _run(emu, case, tmp_path, "boxinf", EMU_CUS=1)
+    xo, so, zo, yo, co, io = O.solve_qp(*data, solver_tol=1e-6)
+    np.testing.assert_array_equal(out["conv"], co)
+    assert np.abs(out["iters"] - io).max() <= 1
+    assert rel_err(out["x"].reshape(B, n), xo).max() < 1e-9
+    fin = np.isfinite(so)
+    np.testing.assert_array_equal(np.isinf(out["s"].reshape(B, mi)), ~fin)
+    assert np.abs(out["z"].reshape(B, mi)[~fin]).max() == 0.0

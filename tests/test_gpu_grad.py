@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle import qp_oracle as O
-from qpax_amd import problems as P
+from tests import problems as P
 from tests._util import rel_err, to_np
 
 torch = pytest.importorskip("torch")
@@ -19,6 +19,10 @@ GCASES = [
     # n = 70: the relaxation's stopping test (residual < 1e-5) can flip by one iteration between two
     # fp64 eliminations on a few problems, which moves the relaxed point by a step of order 1e-6
     ("odd-f64", lambda: P.random_feasible(16, 70, 3, 140, seed=8), np.float64, 1e-5),
+    # box rows (G = [G_dense; I; -I]): the wavefront kernels' diagonal/gather path for the +-e_j rows
+    ("box20-f32", lambda: P.box_qp(32, 20, 3, 10, seed=5), np.float32, 1e-3),
+    ("box50-f64", lambda: P.box_qp(32, 50, 8, 20, seed=7), np.float64, 1e-7),
+    ("nnls-f64", lambda: P.nnls(64, seed=1), np.float64, 1e-7),
 ]
 
 

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import qp_oracle as O
-from qpax_amd import problems as P
+from tests import problems as P
 from tests._util import rel_err, to_np
 
 torch = pytest.importorskip("torch")

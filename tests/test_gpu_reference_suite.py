@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from qpax_amd import problems as P
+from tests import problems as P
 from tests._util import to_np
 
 torch = pytest.importorskip("torch")

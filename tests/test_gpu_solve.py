@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle import qp_oracle as O
-from qpax_amd import problems as P
+from tests import problems as P
 from tests._util import conditioning_floor, rel_err, to_np
 
 torch = pytest.importorskip("torch")
@@ -38,6 +38,12 @@ CASES = [
     ("w64x4-f32", lambda: P.random_feasible(64, 40, 8, 200, seed=37), np.float32, 1e-5, 1e-4),
     ("w64x4-odd-me0-f64", lambda: P.random_feasible(64, 43, 0, 201, seed=38), np.float64, 1e-6, 1e-9),
     ("w64x4-strip52-f64", lambda: P.random_feasible(64, 52, 8, 256, seed=39), np.float64, 1e-6, 1e-9),
+    # box rows below general rows (G = [G_dense; I; -I], the layout of the reference's QPTEST):
+    # the dense streams stop at the last general row, the +-e_j rows enter as a diagonal and gathers
+    ("box20-f32", lambda: P.box_qp(64, 20, 3, 10, seed=5), np.float32, 1e-5, 1e-4),
+    ("box40-dense0-f64", lambda: P.box_qp(64, 40, 4, 0, seed=6), np.float64, 1e-6, 1e-9),
+    ("box50-f64", lambda: P.box_qp(64, 50, 8, 20, seed=7), np.float64, 1e-6, 1e-9),
+    ("box24-inf-f64", lambda: P.box_qp(64, 24, 2, 6, seed=8, inf_frac=0.3), np.float64, 1e-6, 1e-9),
 ]
 
 
@@ -162,6 +168,15 @@ def test_split_wait_giving_up_is_reported(monkeypatch):
         pdip.solve_qp_relaxed_batch(bt, 1e-6, 1e-3)
     monkeypatch.setattr(pdip, "check_relaxed", lambda riters: None)
     out = [to_np(t) for t in pdip.solve_qp_relaxed_batch(bt, 1e-6, 1e-3)]
+    # under autograd (no read-back in the forward) the problems whose relaxation did not start get
+    # NaN cotangents, the others finite ones
+    import qpax_amd
+
+    ts = [torch.tensor(np.asarray(v), dtype=torch.float64, device="cuda", requires_grad=True) for v in data]
+    x = qpax_amd.solve_qp_primal(*ts, solver_tol=1e-6, target_kappa=1e-3)
+    ((x - 1.0) ** 2).sum().backward()
+    gq = to_np(ts[1].grad)
+    assert np.isnan(gq[F:]).all() and np.isfinite(gq[:F]).all()
     monkeypatch.delenv("QPX_SPLIT_SPIN")
     good = [to_np(t) for t in good]
     xr, sr, zr, yr, rconv, riters = out[6], out[7], out[8], out[9], out[10], out[11]

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import qp_oracle as O
-from qpax_amd import problems as P
+from tests import problems as P
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 

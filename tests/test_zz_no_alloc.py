@@ -8,7 +8,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from qpax_amd import problems as P
+from tests import problems as P
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
