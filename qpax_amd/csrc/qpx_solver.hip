@@ -124,6 +124,184 @@ __global__ __launch_bounds__(NT) void k_step(Args<TS> a, int relax) {
   }
 }
 
+// ------------------------- plain-mode stage kernels with LDS images (n < 64) -----------------
+// For the sizes below the matrix-core path the plain stages work in LDS: each kernel stages the
+// caller's G (read in its input precision with coalesced loads, rows with infinite h zeroed as
+// pdip.py:176 does) and what else it reuses -- symmetrised Q and A for the residuals, the factor
+// for the solves -- into the workgroup's LDS, runs the same stage code on LDS pointers (Cholesky
+// of H, W = L^-1 A^T, F = A H^-1 A^T and the triangular sweeps among them), and writes back what
+// the next stage reads.  Vectors move with them.
+__host__ __device__ inline int64_t lds_words(int64_t k) { return round_up(k > 0 ? k : 1, 2); }
+__host__ __device__ inline int plain_lds_bytes(int n, int me, int mi) {
+  const int64_t G = lds_words((int64_t)mi * n), H = lds_words((int64_t)n * n), W = lds_words((int64_t)n * me),
+                F = lds_words((int64_t)me * me), vn = lds_words(n), vm = lds_words(mi), ve = lds_words(me);
+  const int64_t resid = G + H + lds_words((int64_t)me * n) + 3 * vn + 4 * vm + 3 * ve;  // G, Q, A; vectors
+  const int64_t factor = G + H + W + F + 2 * vn + 2 * ve + vm;                          // + hdiag/hrcp, fdiag/frcp, D
+  const int64_t solve = G + H + W + F + 2 * vn + 2 * ve + vm + 3 * vn + 7 * vm + 4 * ve;
+  int64_t w = resid > factor ? resid : factor;
+  w = w > solve ? w : solve;
+  return (int)(8 * w);
+}
+// two workgroups per CU beside the stage code's static LDS (block reductions, the blocked sweeps'
+// right-hand-side buffers)
+constexpr int kPlainLdsMax = 74 * 1024;
+__host__ __device__ inline bool plain_lds(int n, int me, int mi) {
+  return n < kMfmaMinN && plain_lds_bytes(n, me, mi) <= kPlainLdsMax;
+}
+
+struct LdsCarve {
+  double* p;
+  __device__ double* take(int64_t k) {
+    double* r = p;
+    p += lds_words(k);
+    return r;
+  }
+};
+template <int NT>
+__device__ __forceinline__ void copy_vec(double* dst, const double* src, int count) {
+  for (int e = threadIdx.x; e < count; e += NT) dst[e] = src[e];
+}
+// G from the caller's array into LDS (float64), rows with infinite h zeroed (0 * G, as stage_load)
+template <int NT, typename TS>
+__device__ __forceinline__ void stage_G_lds(double* dst, const TS* G, const double* hinf, int mi, int n) {
+  for (int e = threadIdx.x; e < mi * n; e += NT) {
+    const int k = e / n;
+    const double g = (double)G[e];
+    dst[e] = hinf[k] != 0.0 ? 0.0 * g : g;
+  }
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_resid_lds(Args<TS> a, int relax) {
+  __shared__ double red[NT / 64];
+  QPX_DYN_LDS(qpx_smem);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0u;
+  const PIdx pi = pidx(a, blockIdx.x);
+  const View g = view_of(a, pi);
+  const bool act = is_active(g, relax);
+  __syncthreads();
+  if (threadIdx.x == 0) g.ctl[kActive] = act ? 1 : 0;
+  if (!act) return;
+  const int n = pi.n, me = pi.me, mi = pi.mi;
+  LdsCarve c{reinterpret_cast<double*>(qpx_smem)};
+  View v = g;
+  v.G = c.take((int64_t)mi * n);
+  v.Q = c.take((int64_t)n * n);
+  v.A = c.take((int64_t)me * n);
+  v.x = c.take(n); v.q = c.take(n); v.r1 = c.take(n);
+  v.s = c.take(mi); v.z = c.take(mi); v.h = c.take(mi); v.r3 = c.take(mi);
+  v.y = c.take(me); v.b = c.take(me); v.r4 = c.take(me);
+  v.r2 = g.r2;
+  stage_G_lds<NT, TS>(v.G, a.G + pi.oG, g.hinf, mi, n);
+  for (int e = threadIdx.x; e < n * n; e += NT) v.Q[e] = (double)a.Q[pi.oQ + e];  // symmetrised below
+  for (int e = threadIdx.x; e < me * n; e += NT) v.A[e] = (double)a.A[pi.oA + e];
+  copy_vec<NT>(v.x, g.x, n); copy_vec<NT>(v.q, g.q, n);
+  copy_vec<NT>(v.s, g.s, mi); copy_vec<NT>(v.z, g.z, mi); copy_vec<NT>(v.h, g.h, mi);
+  copy_vec<NT>(v.y, g.y, me); copy_vec<NT>(v.b, g.b, me);
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += NT) {  // Q symmetrised as pdip.py:223 (same bits as stage_load)
+    const int i = e / n, j = e - i * n;
+    if (j < i) {
+      const double sym = 0.5 * (v.Q[e] + v.Q[j * n + i]);
+      v.Q[e] = sym;
+      v.Q[j * n + i] = sym;
+    }
+  }
+  __syncthreads();
+  const int conv = stage_residuals<NT>(v, relax ? a.kappa : 0.0, a.tol, red);
+  copy_vec<NT>(g.r1, v.r1, n);
+  copy_vec<NT>(g.r3, v.r3, mi);
+  copy_vec<NT>(g.r4, v.r4, me);
+  if (threadIdx.x == 0) g.ctl[5] = conv;  // entry flag, committed by k_step
+}
+
+// all_problems: the backward's factorisation, which every problem takes
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_factor_lds(Args<TS> a, int all_problems) {
+  QPX_DYN_LDS(qpx_smem);
+  const PIdx pi = pidx(a, blockIdx.x);
+  const View g = view_of(a, pi);
+  if (!all_problems && !g.ctl[kActive]) return;
+  const int n = pi.n, me = pi.me, mi = pi.mi;
+  LdsCarve c{reinterpret_cast<double*>(qpx_smem)};
+  View v = g;
+  v.G = c.take((int64_t)mi * n);
+  v.H = c.take((int64_t)n * n);
+  v.W = c.take((int64_t)n * me);
+  v.F = c.take((int64_t)me * me);
+  v.hdiag = c.take(n); v.hrcp = c.take(n); v.fdiag = c.take(me); v.frcp = c.take(me);
+  v.D = c.take(mi);
+  stage_G_lds<NT, TS>(v.G, a.G + pi.oG, g.hinf, mi, n);
+  __syncthreads();
+  stage_factor<NT>(v, kDzs);
+  for (int e = threadIdx.x; e < n * n; e += NT) g.H[e] = v.H[e];
+  copy_vec<NT>(g.W, v.W, n * me);
+  copy_vec<NT>(g.F, v.F, me * me);
+  copy_vec<NT>(g.hrcp, v.hrcp, n);
+  copy_vec<NT>(g.frcp, v.frcp, me);
+  copy_vec<NT>(g.D, v.D, mi);
+}
+
+// Solve stage on LDS: the right-hand side (a1..a4) and the outputs are the View's vectors named by
+// `which` (0: affine direction into dsa/dza, 1: full direction into ds/dz; dx, dy either way).
+template <int NT, typename TS>
+__device__ void solve_lds(const Args<TS>& a, const PIdx& pi, const View& g, int which) {
+  QPX_DYN_LDS(qpx_smem);
+  const int n = pi.n, me = pi.me, mi = pi.mi;
+  LdsCarve c{reinterpret_cast<double*>(qpx_smem)};
+  View v = g;
+  v.G = c.take((int64_t)mi * n);
+  v.H = c.take((int64_t)n * n);
+  v.W = c.take((int64_t)n * me);
+  v.F = c.take((int64_t)me * me);
+  v.hdiag = c.take(n); v.hrcp = c.take(n); v.fdiag = c.take(me); v.frcp = c.take(me);
+  v.D = c.take(mi);
+  v.r1 = c.take(n); v.u = c.take(n); v.dx = c.take(n);
+  v.r2 = c.take(mi); v.r3 = c.take(mi); v.z = c.take(mi); v.s = c.take(mi);
+  double* ds = c.take(mi);
+  double* dz = c.take(mi);
+  v.r4 = c.take(me); v.dy = c.take(me); v.t = c.take(me);
+  stage_G_lds<NT, TS>(v.G, a.G + pi.oG, g.hinf, mi, n);
+  for (int e = threadIdx.x; e < n * n; e += NT) v.H[e] = g.H[e];
+  copy_vec<NT>(v.W, g.W, n * me);
+  copy_vec<NT>(v.F, g.F, me * me);
+  copy_vec<NT>(v.hrcp, g.hrcp, n);
+  copy_vec<NT>(v.frcp, g.frcp, me);
+  copy_vec<NT>(v.D, g.D, mi);
+  copy_vec<NT>(v.r1, g.r1, n);
+  copy_vec<NT>(v.r2, g.r2, mi); copy_vec<NT>(v.r3, g.r3, mi);
+  copy_vec<NT>(v.z, g.z, mi); copy_vec<NT>(v.s, g.s, mi);
+  copy_vec<NT>(v.r4, g.r4, me);
+  __syncthreads();
+  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, ds, dz);
+  copy_vec<NT>(g.dx, v.dx, n);
+  copy_vec<NT>(g.dy, v.dy, me);
+  copy_vec<NT>(which ? g.ds : g.dsa, ds, mi);
+  copy_vec<NT>(which ? g.dz : g.dza, dz, mi);
+}
+
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_solve_lds(Args<TS> a, int affine) {
+  const PIdx pi = pidx(a, blockIdx.x);
+  const View g = view_of(a, pi);
+  if (!g.ctl[kActive]) return;
+  solve_lds<NT, TS>(a, pi, g, affine ? 0 : 1);
+}
+
+template <int NT, typename TS>
+__device__ void bwd_write(const Args<TS>& a, const View& v, int64_t p);
+
+// The backward's solve (right-hand side (-dl/dx, 0, 0, 0) from bwd_load) on LDS, then the
+// cotangents from the directions it wrote back.
+template <int NT, typename TS>
+__global__ __launch_bounds__(NT) void k_bwd_solve_lds(Args<TS> a) {
+  const PIdx pi = pidx(a, blockIdx.x);
+  const View g = view_of(a, pi);
+  solve_lds<NT, TS>(a, pi, g, 1);
+  __syncthreads();
+  bwd_write<NT, TS>(a, g, blockIdx.x);
+}
+
 // Post-processing of solve_qp (pdip.py:243-251) and start of the relaxation.
 template <int NT, typename TS>
 __device__ void finish_solve(const Args<TS>& a, const View& v, int64_t p) {
@@ -233,10 +411,9 @@ __device__ void bwd_load(const Args<TS>& a, const View& v, int64_t p) {
 
 // Solve with rhs (-dl_dx, 0, 0, 0) and form the cotangents (optnet_derivatives, diff_qp.py:8-17).
 template <int NT, typename TS>
-__device__ void bwd_solve_write(const Args<TS>& a, const View& v, int64_t p) {
+__device__ void bwd_write(const Args<TS>& a, const View& v, int64_t p) {
   const PIdx pi = pidx(a, p);
   const int n = pi.n, me = pi.me, mi = pi.mi, tid = threadIdx.x;
-  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
   // dz here is dlam_tilde; dlam = dlam_tilde / lam
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e - i * n;
@@ -259,6 +436,12 @@ __device__ void bwd_solve_write(const Args<TS>& a, const View& v, int64_t p) {
     const double lam = v.z[k], dlam = v.dz[k] / lam;
     a.dh[pi.oh + k] = (TS)(-lam * dlam);
   }
+}
+
+template <int NT, typename TS>
+__device__ void bwd_solve_write(const Args<TS>& a, const View& v, int64_t p) {
+  stage_solve<NT>(v, v.r1, v.r2, v.r3, v.r4, -1.0, v.dx, v.ds, v.dz);
+  bwd_write<NT, TS>(a, v, p);
 }
 
 template <int NT, typename TS>
@@ -627,17 +810,30 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
   if (!host) return fail(QPX_ERR_HIP, "hipHostMalloc of the plain-mode counter failed");
   hipLaunchKernelGGL((qpx::k_setup<NT, TS>), grid, block, lds, st, a);
   QPX_HIP(hipGetLastError());
+  const bool in_lds = qpx::plain_lds(a.n, a.me, a.mi);
+  const int plds = qpx::plain_lds_bytes(a.n, a.me, a.mi);
   for (int phase = 0; phase < (relax ? 2 : 1); ++phase) {
     const int rl = phase;
     for (int it = 0; it < qpx::kMaxIters; ++it) {
-      hipLaunchKernelGGL((qpx::k_resid<NT, TS>), grid, block, lds, st, a, rl);
-      hipLaunchKernelGGL((qpx::k_factor<NT, TS>), grid, block, lds, st, a);
-      if (!rl) {
-        hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, lds, st, a, 1);
-        hipLaunchKernelGGL((qpx::k_center<NT, TS>), grid, block, lds, st, a);
+      if (in_lds) {
+        hipLaunchKernelGGL((qpx::k_resid_lds<NT, TS>), grid, block, plds, st, a, rl);
+        hipLaunchKernelGGL((qpx::k_factor_lds<NT, TS>), grid, block, plds, st, a, 0);
+        if (!rl) {
+          hipLaunchKernelGGL((qpx::k_solve_lds<NT, TS>), grid, block, plds, st, a, 1);
+          hipLaunchKernelGGL((qpx::k_center<NT, TS>), grid, block, 0, st, a);
+        }
+        hipLaunchKernelGGL((qpx::k_solve_lds<NT, TS>), grid, block, plds, st, a, 0);
+        hipLaunchKernelGGL((qpx::k_step<NT, TS>), grid, block, 0, st, a, rl);
+      } else {
+        hipLaunchKernelGGL((qpx::k_resid<NT, TS>), grid, block, lds, st, a, rl);
+        hipLaunchKernelGGL((qpx::k_factor<NT, TS>), grid, block, lds, st, a);
+        if (!rl) {
+          hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, lds, st, a, 1);
+          hipLaunchKernelGGL((qpx::k_center<NT, TS>), grid, block, lds, st, a);
+        }
+        hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, lds, st, a, 0);
+        hipLaunchKernelGGL((qpx::k_step<NT, TS>), grid, block, lds, st, a, rl);
       }
-      hipLaunchKernelGGL((qpx::k_solve<NT, TS>), grid, block, lds, st, a, 0);
-      hipLaunchKernelGGL((qpx::k_step<NT, TS>), grid, block, lds, st, a, rl);
       QPX_HIP(hipGetLastError());
       QPX_HIP(hipMemcpyAsync(host, a.counter, sizeof(int), hipMemcpyDeviceToHost, st));
       QPX_HIP(hipStreamSynchronize(st));
@@ -668,6 +864,11 @@ qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
       return QPX_OK;
     }
     hipLaunchKernelGGL((qpx::k_fused_bwd<NT, TS>), grid, block, lds, st, a);
+  } else if (qpx::plain_lds(a.n, a.me, a.mi)) {
+    const int plds = qpx::plain_lds_bytes(a.n, a.me, a.mi);
+    hipLaunchKernelGGL((qpx::k_bwd_load<NT, TS>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((qpx::k_factor_lds<NT, TS>), grid, block, plds, st, a, 1);
+    hipLaunchKernelGGL((qpx::k_bwd_solve_lds<NT, TS>), grid, block, plds, st, a);
   } else {
     hipLaunchKernelGGL((qpx::k_bwd_load<NT, TS>), grid, block, lds, st, a);
     hipLaunchKernelGGL((qpx::k_bwd_factor<NT, TS>), grid, block, lds, st, a);

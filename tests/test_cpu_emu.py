@@ -102,14 +102,18 @@ def test_emulated_wave_uniform(emu, tmp_path, tag, n, me, mi, dt, cus):
     _check(tag, data, out, dt, tol, kappa, B, (n, me, mi))
 
 
-def test_emulated_plain_mode(emu, tmp_path):
-    """The per-stage kernels of the plain mode (host-checked loop) at a matrix-core size."""
-    B, n, me, mi, tol, kappa = 2, 66, 6, 90, 1e-6, 1e-3
-    data = list(P.random_feasible(B, n, me, mi, seed=12))
-    case = dict(kind="uniform", dtype="float64", mode="plain", tol=tol, kappa=kappa,
+@pytest.mark.parametrize("n,me,mi,dt", [(66, 6, 90, np.float64), (50, 10, 100, np.float32), (20, 5, 40, np.float64)],
+                         ids=["n66-global", "n50-lds-f32", "n20-lds-wave"])
+def test_emulated_plain_mode(emu, tmp_path, n, me, mi, dt):
+    """The per-stage kernels of the plain mode (host-checked loop): at n >= 64 on the workspace with
+    matrix-core G^T D G, below that with the caller's G and the factor staged in LDS (n = 20: one
+    wavefront per problem)."""
+    B, tol, kappa = 2, 1e-6, 1e-3
+    data = [np.asarray(v, dt).astype(np.float64) for v in P.random_feasible(B, n, me, mi, seed=12)]
+    case = dict(kind="uniform", dtype=np.dtype(dt).name, mode="plain", tol=tol, kappa=kappa,
                 **dict(zip("Q q A b G h".split(), data)))
-    out = _run(emu, case, tmp_path, "plain")
-    _check("plain-n66", data, out, np.float64, tol, kappa, B, (n, me, mi))
+    out = _run(emu, case, tmp_path, f"plain{n}")
+    _check(f"plain-n{n}", data, out, dt, tol, kappa, B, (n, me, mi))
 
 
 def _ragged_case(probs, dt, tol, kappa, tune=0):
