@@ -216,8 +216,8 @@ QPX_API qpx_status qpx_diff_qp_elastic(int32_t dtype, int32_t mode, int64_t batc
  * plan owns, forked from the caller's stream and joined back into it, so the call is still ordered
  * on the caller's stream as one unit (a captured call is a graph with one branch per lane).  The
  * environment variable QPX_RAGGED_LANES, read at plan creation, caps the lanes (1: every bucket on
- * the caller's stream).  One plan serves one call at a time: concurrent calls on one plan from
- * several host threads race on its fork/join events. */
+ * the caller's stream).  Calls on one plan from several host threads are serialised while they
+ * enqueue (a mutex per plan), so they may share a plan; each needs its own workspace. */
 typedef struct qpx_plan qpx_plan;
 
 /* Buckets the problems by kernel variant, orders each bucket longest-first and uploads the tables

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -82,6 +83,9 @@ struct qpx_plan {
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kSide] = {};
+  // held while a call enqueues its fork, bucket launches and joins, so that calls from several
+  // host threads on one plan do not interleave their event records and waits
+  mutable std::mutex mu;
 };
 
 namespace {
@@ -220,6 +224,7 @@ qpx_status fan_out(const qpx_plan* p, hipStream_t st, L&& launch) {
 
 template <typename TS>
 qpx_status run_fwd(const qpx_plan* p, const FwdIO& io, bool relax, void* ws, hipStream_t st, int only = -1) {
+  std::lock_guard<std::mutex> lock(p->mu);
   QPX_HIP(reset_queues(p, ws, st));
   auto launch = [&](int i, hipStream_t s) {
     const Bucket& b = p->buckets[i];
@@ -236,6 +241,7 @@ qpx_status run_fwd(const qpx_plan* p, const FwdIO& io, bool relax, void* ws, hip
 
 template <typename TS>
 qpx_status run_bwd(const qpx_plan* p, const BwdIO& io, void* ws, hipStream_t st) {
+  std::lock_guard<std::mutex> lock(p->mu);
   QPX_HIP(reset_queues(p, ws, st));
   return fan_out(p, st, [&](int i, hipStream_t s) {
     const Bucket& b = p->buckets[i];

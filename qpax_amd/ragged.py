@@ -126,13 +126,19 @@ class RaggedPlan:
         _lib.check(st, "qpx_plan_solve")
         return tuple(out[:6])
 
-    def solve_relaxed(self, ops, solver_tol=1e-5, target_kappa=1e-3, out=None, tune=False):
+    def new_workspace(self):
+        """A workspace of this plan's size, for calls that run at the same time as others on the
+        plan (each concurrent call needs its own)."""
+        return torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=self.device)
+
+    def solve_relaxed(self, ops, solver_tol=1e-5, target_kappa=1e-3, out=None, tune=False, workspace=None):
         """solve_qp then relax_qp (the forward of solve_qp_primal); returns the 12 packed outputs."""
         self._check(ops)
         out = out if out is not None else self.outputs(relax=True)
+        ws = workspace if workspace is not None else self.workspace
         fn = self._lib.qpx_plan_tune if tune else self._lib.qpx_plan_solve_relaxed
         st = fn(self._h, *[_p(t) for t in ops], float(solver_tol), float(target_kappa), *[_p(t) for t in out],
-                _p(self.workspace), self.ws_bytes, stream_of(self.device))
+                _p(ws), self.ws_bytes, stream_of(self.device))
         _lib.check(st, "qpx_plan_tune" if tune else "qpx_plan_solve_relaxed")
         return tuple(out)
 
@@ -140,14 +146,15 @@ class RaggedPlan:
         """Autotunes every bucket's launch shape on these operands (synchronises)."""
         return self.solve_relaxed(ops, solver_tol, target_kappa, tune=True)
 
-    def diff(self, ops, relaxed, dl_dx, exact_dG=False, out=None):
+    def diff(self, ops, relaxed, dl_dx, exact_dG=False, out=None, workspace=None):
         """Backward of solve_qp_primal at the relaxed point (xr, sr, zr, yr); returns packed
         (dQ, dq, dA, db, dG, dh)."""
         self._check(ops)
         out = out if out is not None else self.grad_outputs()
+        ws = workspace if workspace is not None else self.workspace
         flags = _lib.QPX_EXACT_DG if exact_dG else 0
         st = self._lib.qpx_plan_diff(self._h, *[_p(t) for t in ops], *[_p(t) for t in relaxed], _p(dl_dx), flags,
-                                     *[_p(t) for t in out], _p(self.workspace), self.ws_bytes,
+                                     *[_p(t) for t in out], _p(ws), self.ws_bytes,
                                      stream_of(self.device))
         _lib.check(st, "qpx_plan_diff")
         return tuple(out)

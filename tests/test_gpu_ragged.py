@@ -237,3 +237,43 @@ def test_plan_rejects_bad_sizes():
         RaggedPlan([4, 0], [0, 0], [3, 3])
     with pytest.raises(RuntimeError, match="invalid sizes"):
         RaggedPlan([4, 5], [0, 0], [3, 0])
+
+
+def test_plan_shared_by_two_threads_and_streams():
+    """Two host threads call one plan at once, each on its own stream with its own workspace: the
+    plan's mutex keeps their fork/join enqueueing apart, and each gets the serial call's bits."""
+    import threading
+
+    probs = P.uneven(24, seed=43, n_lo=8, n_hi=96)
+    print("[case] one plan, two threads and streams, B=24", flush=True)
+    plan, ops = _plan(probs, torch.float64)
+    ref = [t.clone() for t in plan.solve_relaxed(ops, 1e-5, 1e-3)]
+    gref = [t.clone() for t in plan.diff(ops, ref[6:10], 2.0 * (ref[0] - 1.0))]
+    torch.cuda.synchronize()
+    results = [None, None]
+    errors = []
+
+    def worker(k):
+        try:
+            s = torch.cuda.Stream()
+            ws = plan.new_workspace()
+            with torch.cuda.stream(s):
+                for _ in range(3):
+                    out = plan.solve_relaxed(ops, 1e-5, 1e-3, workspace=ws)
+                    g = plan.diff(ops, out[6:10], 2.0 * (out[0] - 1.0), workspace=ws)
+            s.synchronize()
+            results[k] = (out, g)
+        except Exception as exc:  # reported by the main thread
+            errors.append(exc)
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    for out, g in results:
+        for a, b in zip(out, ref):
+            assert torch.equal(a, b)
+        for a, b in zip(g, gref):
+            assert torch.equal(a, b)
