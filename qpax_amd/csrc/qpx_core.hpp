@@ -32,6 +32,21 @@ __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a 
 #define QPX_PIN_V(x) asm volatile("" : "+v"(x))
 #define QPX_PIN_S(x) asm volatile("" : "+s"(x))
 #endif
+// A pointer's global-memory view: loads through it are `global_load`s, counted by vmcnt alone (a
+// generic pointer's `flat_load`s also count in lgkmcnt, so every LDS wait would wait for them too).
+#ifndef QPX_GLOBAL_AS
+#define QPX_GLOBAL_AS __attribute__((address_space(1)))
+#endif
+template <typename T>
+__device__ __forceinline__ const QPX_GLOBAL_AS T* as_global(const T* p) {
+  return (const QPX_GLOBAL_AS T*)p;
+}
+// Lane `lane`'s value of v (a uniform result).
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
 // The workgroup's dynamic LDS as `unsigned char* name`.
 #ifndef QPX_DYN_LDS
 #define QPX_DYN_LDS(name) extern __shared__ __align__(16) unsigned char name[]
@@ -304,16 +319,22 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
     const int e = tid + NT * u;
     dst[u] = e < csz ? (e / n) * ldl + (e - (e / n) * n) : -1;
   }
-  double reg[EPT], dreg = 0.0;
+  // the next chunk in flight: raw elements (converted when stored) loaded unconditionally through
+  // clamped indices, so that every load issues before the first wait
+  GT reg[EPT];
+  double dreg = 0.0;
+  const QPX_GLOBAL_AS GT* Gg = as_global(G);
+  const int64_t last = (int64_t)mi * n - 1;
   auto fetch = [&](int k0) {
     if constexpr (!kPf) return;
-    const int64_t base = (int64_t)k0 * n, total = (int64_t)mi * n;
+    const int64_t base = (int64_t)k0 * n;
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
       const int64_t g = base + tid + NT * u;
-      reg[u] = (dst[u] >= 0 && g < total) ? (double)G[g] : 0.0;
+      reg[u] = Gg[g < last ? g : last];
     }
-    dreg = (tid < kGtdgRows && k0 + tid < mi) ? v.D[k0 + tid] : 0.0;
+    const int kd = k0 + (tid & (kGtdgRows - 1));
+    dreg = as_global(v.D)[kd < mi ? kd : mi - 1];
   };
   for (int base = 0; base < T; base += W * kBatch) {
     int tI[kBatch], tJ[kBatch];
@@ -329,9 +350,16 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
       tJ[b] = ok[b] ? idx - I * (I + 1) / 2 : 0;
     }
     const int nb = (T - base - w + W - 1) / W;  // valid slots of this wavefront in the pass
+    // the accumulators start from Q's tiles (clamped indices: every load issues unconditionally)
     d4v acc[kBatch];
+    const QPX_GLOBAL_AS double* Qg = as_global(v.Q);
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) acc[b] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < kBatch; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * tI[b] + 4 * r + rq, j = 16 * tJ[b] + cq;
+        acc[b][r] = Qg[(i < n ? i : n - 1) * n + (j < n ? j : n - 1)];
+      }
     __syncthreads();
     for (int e = tid; e < kGtdgRows * (ldl - n); e += NT) {
       const int r = e / (ldl - n);
@@ -341,10 +369,11 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
     for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
       __syncthreads();
       if constexpr (kPf) {
+        const int64_t base = (int64_t)k0 * n;
 #pragma unroll
         for (int u = 0; u < EPT; ++u)
-          if (dst[u] >= 0) chunk[dst[u]] = reg[u];
-        if (tid < kGtdgRows) cD[tid] = dreg;
+          if (dst[u] >= 0) chunk[dst[u]] = base + tid + NT * u <= last ? (double)reg[u] : 0.0;
+        if (tid < kGtdgRows) cD[tid] = k0 + tid < mi ? dreg : 0.0;
       } else {
         for (int r = 0; r < kGtdgRows; ++r) {
           const int k = k0 + r;
@@ -375,7 +404,7 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * tI[b] + 4 * r + rq, j = 16 * tJ[b] + cq;
-        if (i < n && j <= i) v.H[i * n + j] = v.Q[i * n + j] + acc[b][r];
+        if (i < n && j <= i) v.H[i * n + j] = acc[b][r];
       }
     }
   }
@@ -392,58 +421,119 @@ __device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rc
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
   const int nt = (m + 15) / 16;
+  const QPX_GLOBAL_AS double* Mg = as_global(M);
   for (int i = tid; i < m; i += NT) diag0[i] = M[i * m + i];
   __syncthreads();
   for (int k = 0; k < nt; ++k) {
     const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16, R = m - c0;
-    for (int e = tid; e < R * 16; e += NT) {
-      const int r = e >> 4, c = e & 15;
-      P[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? M[(c0 + r) * m + c0 + c] : 0.0;
+    // panel: rows c0.., columns c0..c0+cw-1 into P (zero above the diagonal and beyond cw); loads
+    // through clamped indices, eight in flight per thread
+    for (int e0 = 0; e0 < R * 16; e0 += 8 * NT) {
+      double val[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + tid + u * NT, r = e >> 4 < R ? e >> 4 : R - 1, c = (e & 15) < cw ? e & 15 : cw - 1;
+        val[u] = Mg[(c0 + r) * m + c0 + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + tid + u * NT, r = e >> 4, c = e & 15;
+        if (r < R) P[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? val[u] : 0.0;
+      }
     }
     __syncthreads();
-    for (int j = 0; j < cw; ++j) {
-      const double d = P[j * kPanelLd + j];
-      const bool ok = d > kPivotRel * fabs(diag0[c0 + j]);
-      const double ljj = ok ? sqrt(d) : kHugePivot;
-      const double inv = 1.0 / ljj;
-      for (int r = j + 1 + tid; r < R; r += NT) P[r * kPanelLd + j] *= inv;
-      __syncthreads();
-      if (tid == 0) { P[j * kPanelLd + j] = ljj; rcp[c0 + j] = inv; }
-      for (int e = tid; e < R * 16; e += NT) {
-        const int r = e >> 4, c = e & 15;
-        if (c > j && c < cw && r >= c) P[r * kPanelLd + c] -= P[r * kPanelLd + j] * P[c * kPanelLd + j];
+    // the 16x16 diagonal block on wavefront 0, lane r holding row r (right-looking, as the scalar
+    // block_cholesky: scale column j by 1/L_jj, then subtract its outer product)
+    if (w == 0) {
+      const int r = l & 15;
+      double p[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) p[c] = P[r * kPanelLd + c];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j < cw) {
+          const double d = readlane_d(p[j], j);
+          const bool ok = d > kPivotRel * fabs(diag0[c0 + j]);
+          const double ljj = ok ? sqrt(d) : kHugePivot;
+          const double inv = 1.0 / ljj;
+          if (r > j) p[j] *= inv;
+          if (r == j) p[j] = ljj;
+          if (l == 0) rcp[c0 + j] = inv;
+#pragma unroll
+          for (int c = j + 1; c < 16; ++c) {
+            const double lcj = readlane_d(p[j], c);
+            if (r >= c && c < cw) p[c] -= p[j] * lcj;
+          }
+        }
       }
-      __syncthreads();
+      if (l < 16)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) P[r * kPanelLd + c] = p[c];
     }
+    __syncthreads();
+    // rows below the block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i]) / L_jj,
+    // the subtractions in the order of the right-looking scalar factorisation
+    for (int r = 16 + tid; r < R; r += NT) {
+      double x[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) x[c] = P[r * kPanelLd + c];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j < cw) {
+          double vj = x[j];
+#pragma unroll
+          for (int i = 0; i < j; ++i) vj -= x[i] * P[j * kPanelLd + i];
+          x[j] = vj * rcp[c0 + j];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) P[r * kPanelLd + c] = x[c];
+    }
+    __syncthreads();
     for (int e = tid; e < R * 16; e += NT) {
       const int r = e >> 4, c = e & 15;
       if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = P[r * kPanelLd + c];
     }
-    // trailing update of tiles (I, J), k < J <= I < nt
+    // trailing update of tiles (I, J), k < J <= I < nt: two tiles per wavefront and step, their
+    // loads issued together through clamped indices
     const int tn = nt - k - 1, T = tn * (tn + 1) / 2;
-    for (int idx = w; idx < T; idx += W) {
-      int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-      while ((a + 1) * (a + 2) / 2 <= idx) ++a;
-      while (a * (a + 1) / 2 > idx) --a;
-      const int I = k + 1 + a, J = k + 1 + (idx - a * (a + 1) / 2);
-      d4v acc;
+    for (int idx0 = w; idx0 < T; idx0 += 2 * W) {
+      int tI[2], tJ[2];
+      d4v acc[2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
-        acc[r] = (i < m && jj < m) ? M[i * m + jj] : 0.0;
-      }
-      const double* pa = P + (16 * (I - k) + cq) * kPanelLd + rq;
-      const double* pb = P + (16 * (J - k) + cq) * kPanelLd + rq;
+      for (int t = 0; t < 2; ++t) {
+        const int idx = idx0 + t * W < T ? idx0 + t * W : T - 1;
+        int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+        while ((a + 1) * (a + 2) / 2 <= idx) ++a;
+        while (a * (a + 1) / 2 > idx) --a;
+        tI[t] = k + 1 + a;
+        tJ[t] = k + 1 + (idx - a * (a + 1) / 2);
 #pragma unroll
-      for (int sl = 0; sl < 4; ++sl) {
-        const bool ia = 16 * (I - k) + cq < R, ib = 16 * (J - k) + cq < R;
-        const double av = ia ? -pa[4 * sl] : 0.0, bv = ib ? pb[4 * sl] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
+          acc[t][r] = Mg[(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+        }
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
-        if (i < m && jj < m) M[i * m + jj] = acc[r];
+      for (int t = 0; t < 2; ++t) {
+        const int ia = 16 * (tI[t] - k) + cq, ib = 16 * (tJ[t] - k) + cq;
+        const double* pa = P + ia * kPanelLd + rq;
+        const double* pb = P + ib * kPanelLd + rq;
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+          const double pav = pa[4 * sl], pbv = pb[4 * sl];
+          const double av = ia < R ? -pav : 0.0, bv = ib < R ? pbv : 0.0;
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        if (idx0 + t * W >= T) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
+          if (i < m && jj < m) M[i * m + jj] = acc[t][r];
+        }
       }
     }
     __syncthreads();
@@ -457,24 +547,42 @@ template <int NT>
 __device__ __forceinline__ void fsub_blocks(const double* L, const double* rcp, int m, double* b) {
   const int tid = threadIdx.x, l = tid & 63;
   const int nt = (m + 15) / 16;
+  const QPX_GLOBAL_AS double* Lg = as_global(L);
+  const QPX_GLOBAL_AS double* rg = as_global(rcp);
   for (int k = 0; k < nt; ++k) {
     const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
     if (tid < 64) {
+      // row l of the diagonal block and the block's 1/L_jj, loaded up front (clamped indices)
+      const int lr = c0 + (l < cw ? l : cw - 1);
+      double lrow[16], rc[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int jc = c0 + (j < cw ? j : cw - 1);
+        lrow[j] = Lg[lr * m + jc];
+        rc[j] = rg[jc];
+      }
       double bi = l < cw ? b[c0 + l] : 0.0;
-      for (int j = 0; j < cw; ++j) {
-        const double yj = __shfl(bi, j) * rcp[c0 + j];
-        const double lv = (l > j && l < cw) ? L[(c0 + l) * m + c0 + j] : 0.0;
-        bi = (l == j) ? yj : bi - lv * yj;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j < cw) {
+          const double yj = readlane_d(bi, j) * rc[j];
+          const double lv = (l > j && l < cw) ? lrow[j] : 0.0;
+          bi = (l == j) ? yj : bi - lv * yj;
+        }
       }
       if (l < cw) b[c0 + l] = bi;
     }
     __syncthreads();
     for (int r = c0 + 16 + tid; r < m; r += NT) {
-      const double* lr = L + r * m + c0;
+      double lv[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) lv[j] = Lg[r * m + c0 + (j < cw ? j : cw - 1)];
       double a0 = 0.0, a1 = 0.0;
-      int j = 0;
-      for (; j + 2 <= cw; j += 2) { a0 += lr[j] * b[c0 + j]; a1 += lr[j + 1] * b[c0 + j + 1]; }
-      if (j < cw) a0 += lr[j] * b[c0 + j];
+#pragma unroll
+      for (int j = 0; j < 16; j += 2) {
+        if (j < cw) a0 += lv[j] * b[c0 + j];
+        if (j + 1 < cw) a1 += lv[j + 1] * b[c0 + j + 1];
+      }
       b[r] -= a0 + a1;
     }
     __syncthreads();
@@ -486,26 +594,42 @@ template <int NT>
 __device__ __forceinline__ void bsub_t_blocks(const double* L, const double* rcp, int m, double* b) {
   const int tid = threadIdx.x, l = tid & 63;
   const int nt = (m + 15) / 16;
+  const QPX_GLOBAL_AS double* Lg = as_global(L);
+  const QPX_GLOBAL_AS double* rg = as_global(rcp);
   for (int k = nt - 1; k >= 0; --k) {
     const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
     if (tid < 64) {
+      // column l of the diagonal block and the block's 1/L_jj, loaded up front (clamped indices)
+      const int lc = c0 + (l < cw ? l : cw - 1);
+      double lcol[16], rc[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int jc = c0 + (j < cw ? j : cw - 1);
+        lcol[j] = Lg[jc * m + lc];
+        rc[j] = rg[jc];
+      }
       double bi = l < cw ? b[c0 + l] : 0.0;
-      for (int j = cw - 1; j >= 0; --j) {
-        const double xj = __shfl(bi, j) * rcp[c0 + j];
-        const double lv = (l < j) ? L[(c0 + j) * m + c0 + l] : 0.0;
-        bi = (l == j) ? xj : bi - lv * xj;
+#pragma unroll
+      for (int j = 15; j >= 0; --j) {
+        if (j < cw) {
+          const double xj = readlane_d(bi, j) * rc[j];
+          const double lv = (l < j) ? lcol[j] : 0.0;
+          bi = (l == j) ? xj : bi - lv * xj;
+        }
       }
       if (l < cw) b[c0 + l] = bi;
     }
     __syncthreads();
     for (int r = tid; r < c0; r += NT) {
+      double lv[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) lv[j] = Lg[(c0 + (j < cw ? j : cw - 1)) * m + r];
       double a0 = 0.0, a1 = 0.0;
-      int j = 0;
-      for (; j + 2 <= cw; j += 2) {
-        a0 += L[(c0 + j) * m + r] * b[c0 + j];
-        a1 += L[(c0 + j + 1) * m + r] * b[c0 + j + 1];
+#pragma unroll
+      for (int j = 0; j < 16; j += 2) {
+        if (j < cw) a0 += lv[j] * b[c0 + j];
+        if (j + 1 < cw) a1 += lv[j + 1] * b[c0 + j + 1];
       }
-      if (j < cw) a0 += L[(c0 + j) * m + r] * b[c0 + j];
       b[r] -= a0 + a1;
     }
     __syncthreads();
@@ -597,9 +721,21 @@ __device__ void block_w_lds(const double* L, const double* rcp, int n, int me, d
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
   const int nt = (n + 15) / 16, ct = (me + 15) / 16;
+  const QPX_GLOBAL_AS double* Lg = as_global(L);
   for (int k = 0; k < nt; ++k) {
     const int c0 = 16 * k, cw = n - c0 < 16 ? n - c0 : 16;
     if (w == 0) {
+      // the diagonal block spread over the wavefront: lane cq + 16 t4 holds L[c0 + cq, c0 + 4 t4 + u]
+      // in lb[u]; L[c0 + i, c0 + j] is then lane i + 16 (j / 4)'s lb[j % 4] (clamped indices)
+      const int ri = c0 + (cq < cw ? cq : cw - 1);
+      double lb[4], rc[16];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = 4 * rq + u;
+        lb[u] = Lg[ri * n + c0 + (j < cw ? j : cw - 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) rc[i] = as_global(rcp)[c0 + (i < cw ? i : cw - 1)];
       for (int cg = 0; cg < me; cg += 64) {
         const int c = cg + l;
         double wj[16];
@@ -607,8 +743,8 @@ __device__ void block_w_lds(const double* L, const double* rcp, int n, int me, d
         for (int i = 0; i < 16; ++i) {
           double wi = (i < cw && c < me) ? Wl[(c0 + i) * me + c] : 0.0;
 #pragma unroll
-          for (int j = 0; j < i; ++j) wi -= (i < cw ? L[(c0 + i) * n + c0 + j] : 0.0) * wj[j];
-          wi *= i < cw ? rcp[c0 + i] : 0.0;
+          for (int j = 0; j < i; ++j) wi -= (i < cw ? readlane_d(lb[j & 3], i + 16 * (j >> 2)) : 0.0) * wj[j];
+          wi *= i < cw ? rc[i] : 0.0;
           wj[i] = wi;
           if (i < cw && c < me) Wl[(c0 + i) * me + c] = wi;
         }
@@ -619,18 +755,22 @@ __device__ void block_w_lds(const double* L, const double* rcp, int n, int me, d
     for (int idx = w; idx < T; idx += W) {
       const int R = k + 1 + idx / ct, C = idx - (idx / ct) * ct;
       const int ia = 16 * R + cq, jb = 16 * C + cq;
+      const int iac = ia < n ? ia : n - 1, jbc = jb < me ? jb : me - 1;
       double av[4], bv[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int kk = 4 * s + rq;  // column of the block
-        av[s] = (ia < n && kk < cw) ? -L[ia * n + c0 + kk] : 0.0;
-        bv[s] = (kk < cw && jb < me) ? Wl[(c0 + kk) * me + jb] : 0.0;
+        const int kc = kk < cw ? kk : cw - 1;
+        const double lv = Lg[iac * n + c0 + kc], wv = Wl[(c0 + kc) * me + jbc];
+        av[s] = (ia < n && kk < cw) ? -lv : 0.0;
+        bv[s] = (kk < cw && jb < me) ? wv : 0.0;
       }
       d4v acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * R + 4 * r + rq;
-        acc[r] = (i < n && jb < me) ? Wl[i * me + jb] : 0.0;
+        const double wv = Wl[(i < n ? i : n - 1) * me + jbc];
+        acc[r] = (i < n && jb < me) ? wv : 0.0;
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);

@@ -34,10 +34,13 @@ __host__ __device__ inline EView carve_elastic(double* base, int n, int me, int 
   e.v = carve(base, n, me, mi);
   double* p = base + view_words(n, me, mi);
   const int64_t st = round_up(mi > 0 ? mi : 1, 2);
-  double** slots[] = {&e.tt, &e.s1, &e.s2, &e.z1, &e.z2, &e.r2, &e.r3, &e.r4e, &e.r5, &e.r6,
-                      &e.dt, &e.ds1, &e.ds2, &e.dz1, &e.dz2, &e.ds1a, &e.ds2a, &e.dz1a, &e.dz2a, &e.p1,
-                      &e.pa3};
-  for (double** s : slots) { *s = p; p += st; }
+  // member by member (taking the members' addresses would keep the whole view in scratch memory)
+  auto take = [&]() { double* r = p; p += st; return r; };
+  e.tt = take(); e.s1 = take(); e.s2 = take(); e.z1 = take(); e.z2 = take();
+  e.r2 = take(); e.r3 = take(); e.r4e = take(); e.r5 = take(); e.r6 = take();
+  e.dt = take(); e.ds1 = take(); e.ds2 = take(); e.dz1 = take(); e.dz2 = take();
+  e.ds1a = take(); e.ds2a = take(); e.dz1a = take(); e.dz2a = take(); e.p1 = take();
+  e.pa3 = take();
   e.pen = 0.0;
   return e;
 }

@@ -143,10 +143,11 @@ __host__ __device__ inline int plain_lds_bytes(int n, int me, int mi) {
   return (int)(8 * w);
 }
 // two workgroups per CU beside the stage code's static LDS (block reductions, the blocked sweeps'
-// right-hand-side buffers)
+// right-hand-side buffers).  n and m_eq below the matrix-core sizes: the tiled factorisation and
+// sweeps (n or m_eq >= 64) read their factor through global-memory pointers.
 constexpr int kPlainLdsMax = 74 * 1024;
 __host__ __device__ inline bool plain_lds(int n, int me, int mi) {
-  return n < kMfmaMinN && plain_lds_bytes(n, me, mi) <= kPlainLdsMax;
+  return n < kMfmaMinN && me < kMfmaMinN && plain_lds_bytes(n, me, mi) <= kPlainLdsMax;
 }
 
 struct LdsCarve {

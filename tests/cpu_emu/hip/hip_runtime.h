@@ -31,6 +31,7 @@
 #define QPX_PIN_V(x) asm volatile("" : "+r"(x))
 #define QPX_PIN_S(x) asm volatile("" : "+r"(x))
 #define QPX_DYN_LDS(name) unsigned char* const name = ::emu::dyn_lds()
+#define QPX_GLOBAL_AS
 
 struct alignas(16) double2 {
   double x, y;
