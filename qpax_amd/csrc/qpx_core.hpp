@@ -236,7 +236,7 @@ enum DMode : int { kDzs = 0, kDunit = 1, kDgiven = 2 };
 constexpr int kMfmaMinN = 64;
 constexpr int kMfmaMaxN = 272;
 constexpr int kGtdgRows = 16;
-constexpr int kGtdgBatch = 16;
+constexpr int kGtdgBatch = 17;
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline int gtdg_stride(int n) { return 16 * (((n + 15) / 16) | 1); }
@@ -299,14 +299,16 @@ __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const d
 // G is read from `G` (the workspace copy, or for the elastic solve the caller's array, whose
 // element type GT widens exactly to the copy's float64 values).
 template <int NT, typename GT>
-__device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT* G) {
+__device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double* H, int n, int mi, double* chunk,
+                                       const GT* G) {
   constexpr int W = NT / 64;
-  // 256-thread blocks prefetch the next chunk into registers under the MFMAs and keep 16
-  // accumulators per wavefront; smaller blocks (n <= 96) load in place and keep 8
+  // 256-thread blocks prefetch the next chunk into registers under the MFMAs and keep 17
+  // accumulators per wavefront (the 136 tiles of n = 256 in two passes); smaller blocks (n <= 96)
+  // load in place and keep 8
   constexpr bool kPf = NT >= 256;
   constexpr int kBatch = kPf ? kGtdgBatch : kGtdgBatch / 2;
   constexpr int EPT = kPf ? (kGtdgRows * (16 * ((kMfmaMaxN / 16) | 1)) + NT - 1) / NT : 1;
-  const int n = v.n, mi = v.mi, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
   const int nt = (n + 15) / 16, ldl = gtdg_stride(n), T = nt * (nt + 1) / 2;
   const int csz = kGtdgRows * n;  // one chunk: kGtdgRows consecutive rows of G (row stride n)
@@ -334,7 +336,7 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
       reg[u] = Gg[g < last ? g : last];
     }
     const int kd = k0 + (tid & (kGtdgRows - 1));
-    dreg = as_global(v.D)[kd < mi ? kd : mi - 1];
+    dreg = as_global(D)[kd < mi ? kd : mi - 1];
   };
   for (int base = 0; base < T; base += W * kBatch) {
     int tI[kBatch], tJ[kBatch];
@@ -352,7 +354,7 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
     const int nb = (T - base - w + W - 1) / W;  // valid slots of this wavefront in the pass
     // the accumulators start from Q's tiles (clamped indices: every load issues unconditionally)
     d4v acc[kBatch];
-    const QPX_GLOBAL_AS double* Qg = as_global(v.Q);
+    const QPX_GLOBAL_AS double* Qg = as_global(Q);
 #pragma unroll
     for (int b = 0; b < kBatch; ++b)
 #pragma unroll
@@ -379,24 +381,29 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
           const int k = k0 + r;
           for (int c = tid; c < n; c += NT) chunk[r * ldl + c] = k < mi ? (double)G[(int64_t)k * n + c] : 0.0;
         }
-        for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? v.D[k0 + r] : 0.0;
+        for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? D[k0 + r] : 0.0;
       }
       __syncthreads();
       if (k0 + kGtdgRows < mi) fetch(k0 + kGtdgRows);  // in flight under the MFMAs
+      // (a full batch, the common case, runs its MFMAs unguarded)
+      auto mfmas = [&](auto full) {
 #pragma unroll
-      for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
-        const double* gr = chunk + (4 * ks + rq) * ldl + cq;
-        const double dk = cD[4 * ks + rq];
-        double a[kBatch], bb[kBatch];
+        for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
+          const double* gr = chunk + (4 * ks + rq) * ldl + cq;
+          const double dk = cD[4 * ks + rq];
+          double a[kBatch], bb[kBatch];
 #pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
-          a[b] = gr[16 * tI[b]] * dk;
-          bb[b] = gr[16 * tJ[b]];
+          for (int b = 0; b < kBatch; ++b) {
+            a[b] = gr[16 * tI[b]] * dk;
+            bb[b] = gr[16 * tJ[b]];
+          }
+#pragma unroll
+          for (int b = 0; b < kBatch; ++b)
+            if (decltype(full)::value || b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
         }
-#pragma unroll
-        for (int b = 0; b < kBatch; ++b)
-          if (b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
-      }
+      };
+      if (nb >= kBatch) mfmas(std::true_type{});
+      else mfmas(std::false_type{});
     }
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
@@ -404,7 +411,7 @@ __device__ __forceinline__ void gtdg_mfma(const View& v, double* chunk, const GT
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * tI[b] + 4 * r + rq, j = 16 * tJ[b] + cq;
-        if (i < n && j <= i) v.H[i * n + j] = acc[b][r];
+        if (i < n && j <= i) H[i * n + j] = acc[b][r];
       }
     }
   }
@@ -833,7 +840,7 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
   __syncthreads();
   if (n >= kMfmaMinN && n <= kMfmaMaxN) {
     QPX_DYN_LDS(qpx_smem);
-    gtdg_mfma<NT>(v, reinterpret_cast<double*>(qpx_smem), G);
+    gtdg_mfma<NT>(v.Q, v.D, v.H, n, mi, reinterpret_cast<double*>(qpx_smem), G);
   } else {
     for (int e = tid; e < n * n; e += NT) {
       const int i = e / n, j = e - i * n;
