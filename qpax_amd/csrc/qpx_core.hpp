@@ -194,30 +194,66 @@ __device__ void block_cholesky(double* M, int m, double* diag0, double* rcp) {
   }
 }
 
-// Forward substitution L y = b for one right-hand side; b is overwritten with y.
+// Forward substitution L y = b for one right-hand side (m < 64: the sizes below the blocked
+// sweeps); b is overwritten with y.  Lane i of wavefront 0 holds b_i and applies column j's update
+// b_i -= L_ij y_j (i > j) in column order, as a column sweep over the block would, with eight
+// columns' entries of L and 1/L_jj loaded ahead (clamped indices).
 template <int NT>
 __device__ void block_fsub_scalar(const double* L, const double* rcp, int m, double* b) {
-  const int tid = threadIdx.x;
-  for (int j = 0; j < m; ++j) {
-    const double yj = b[j] * rcp[j];
-    __syncthreads();
-    if (tid == 0) b[j] = yj;
-    for (int i = j + 1 + tid; i < m; i += NT) b[i] -= L[i * m + j] * yj;
-    __syncthreads();
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x, lr = l < m ? l : m - 1;
+    double bi = l < m ? b[l] : 0.0;
+    for (int j0 = 0; j0 < m; j0 += 8) {
+      double lv[8], rc[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = j0 + t < m ? j0 + t : m - 1;
+        lv[t] = L[lr * m + j];
+        rc[t] = rcp[j];
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = j0 + t;
+        if (j < m) {
+          const double yj = readlane_d(bi, j) * rc[t];
+          bi = (l == j) ? yj : (l > j ? bi - lv[t] * yj : bi);
+        }
+      }
+    }
+    if (l < m) b[l] = bi;
   }
+  __syncthreads();
 }
 
-// Back substitution L^T x = b; b is overwritten with x.
+// Back substitution L^T x = b (m < 64); b is overwritten with x.  Lane i of wavefront 0 holds b_i;
+// columns from the last, b_i -= L_ji x_j for i < j.
 template <int NT>
 __device__ void block_bsub_t_scalar(const double* L, const double* rcp, int m, double* b) {
-  const int tid = threadIdx.x;
-  for (int j = m - 1; j >= 0; --j) {
-    const double xj = b[j] * rcp[j];
-    __syncthreads();
-    if (tid == 0) b[j] = xj;
-    for (int i = tid; i < j; i += NT) b[i] -= L[j * m + i] * xj;
-    __syncthreads();
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x, lc = l < m ? l : m - 1;
+    double bi = l < m ? b[l] : 0.0;
+    for (int j0 = m - 1; j0 >= 0; j0 -= 8) {
+      double lv[8], rc[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = j0 - t >= 0 ? j0 - t : 0;
+        lv[t] = L[j * m + lc];
+        rc[t] = rcp[j];
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = j0 - t;
+        if (j >= 0) {
+          const double xj = readlane_d(bi, j) * rc[t];
+          bi = (l == j) ? xj : (l < j ? bi - lv[t] * xj : bi);
+        }
+      }
+    }
+    if (l < m) b[l] = bi;
   }
+  __syncthreads();
 }
 
 // --------------------------------------------------------------------------------------------
@@ -276,11 +312,48 @@ __device__ __forceinline__ double sum16(double v) {
 }
 
 // out[k] = sum_j M[k n + j] x[j] for every row k < rows: 16 lanes per row read it coalesced and
-// meet in a DPP reduction (the large-size form of the row products, n >= kMfmaMinN).
+// meet in a DPP reduction (the large-size form of the row products, n >= kMfmaMinN).  Up to
+// kMfmaMaxN columns a lane keeps its slice of x (columns cq, cq + 16, ...) in registers and each
+// wavefront has two groups of four rows' loads in flight (clamped indices); wider rows loop.
 template <int NT, typename MT>
 __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const double* x, double* out) {
   constexpr int W = NT / 64;
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, rq = l >> 4, cq = l & 15;
+  constexpr int S = (kMfmaMaxN + 15) / 16;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const QPX_GLOBAL_AS MT* Mg = as_global(M);
+  if (n <= kMfmaMaxN) {
+    double xs[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+      const int j = cq + 16 * u;
+      xs[u] = j < n ? x[j] : 0.0;
+    }
+    for (int kb = 4 * w; kb < rows; kb += 8 * W) {
+      MT mv[2][S];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int k0 = kb + 4 * W * g + rq, k = k0 < rows ? k0 : rows - 1;
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+          const int j = cq + 16 * u;
+          mv[g][u] = Mg[(int64_t)k * n + (j < n ? j : n - 1)];
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < S; u += 2) {
+          if (cq + 16 * u < n) a0 += (double)mv[g][u] * xs[u];
+          if (u + 1 < S && cq + 16 * (u + 1) < n) a1 += (double)mv[g][u + 1] * xs[u + 1];
+        }
+        const double v = sum16(a0 + a1);
+        const int k = kb + 4 * W * g + rq;
+        if (cq == 0 && k < rows) out[k] = v;
+      }
+    }
+    return;
+  }
   for (int kb = 4 * w; kb < rows; kb += 4 * W) {
     const int k = kb + rq;
     const MT* mr = M + (int64_t)(k < rows ? k : rows - 1) * n;
@@ -294,6 +367,67 @@ __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const d
     const double v = sum16(a0 + a1);
     if (cq == 0 && k < rows) out[k] = v;
   }
+}
+
+// Column slots per lane of the column sums in the NT-thread generic kernels (n <= 96 runs on 128
+// threads, n <= kMfmaMaxN on 256), and the LDS they take for the partial sums: within the
+// dynamic LDS of those kernels (the G^T D G chunk, free outside the factorisation) for n >= 64.
+template <int NT>
+constexpr int colsum_slots() { return NT >= 256 ? (kMfmaMaxN + 63) / 64 : 2; }
+template <int NT>
+constexpr int colsum_lds_bytes() { return 8 * (NT / 64) * 64 * colsum_slots<NT>(); }
+static_assert(colsum_lds_bytes<128>() <= 8 * (kGtdgRows * 80 + kGtdgRows), "column sums exceed the n = 64 chunk");
+static_assert(colsum_lds_bytes<256>() <= 8 * (kGtdgRows * 112 + kGtdgRows), "column sums exceed the n = 97 chunk");
+
+// Column sums acc[u] += sum_k M[k, c] v[k] for the columns c = l + 64 u (< cols <= 64 CPL) of this
+// lane: the wavefronts take rows w, w + W, ..., eight rows' loads in flight per lane (clamped
+// indices; rows past the end add nothing).  Partial sums per wavefront: finish with colsum_finish.
+template <int NT, int CPL, typename T>
+__device__ __forceinline__ void colsum_acc(const T* M, int rows, int cols, const double* v, double (&acc)[CPL]) {
+  constexpr int W = NT / 64;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const QPX_GLOBAL_AS T* Mg = as_global(M);
+  for (int k0 = w; k0 < rows; k0 += 8 * W) {
+    T m[8][CPL];
+    double vk[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int k = k0 + r * W < rows ? k0 + r * W : rows - 1;
+      vk[r] = v[k];
+#pragma unroll
+      for (int u = 0; u < CPL; ++u) {
+        const int c = l + 64 * u;
+        if (64 * u < cols) m[r][u] = Mg[(int64_t)k * cols + (c < cols ? c : cols - 1)];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (k0 + r * W < rows)
+#pragma unroll
+        for (int u = 0; u < CPL; ++u)
+          if (64 * u < cols) acc[u] += (double)m[r][u] * vk[r];
+  }
+}
+
+// out[c] = init[c] + sg * (sum of the wavefronts' partial column sums), summed in wavefront order
+// through `part` (LDS, NT / 64 x 64 CPL doubles); the sum of |out| entries' maximum is returned to
+// every thread when want_max.  Contains barriers.
+template <int NT, int CPL>
+__device__ __forceinline__ void colsum_finish(const double (&acc)[CPL], int cols, const double* init, double si,
+                                              double sg, double* out, double* part) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < CPL; ++u) part[w * 64 * CPL + l + 64 * u] = acc[u];
+  __syncthreads();
+  for (int c = tid; c < cols; c += NT) {
+    double s = part[c];
+#pragma unroll
+    for (int ww = 1; ww < W; ++ww) s += part[ww * 64 * CPL + c];
+    out[c] = si * init[c] + sg * s;
+  }
+  __syncthreads();
 }
 
 // G is read from `G` (the workspace copy, or for the elastic solve the caller's array, whose
@@ -920,12 +1054,19 @@ __device__ void reduced_solve(const View& v, const double* v4, double* dx) {
   const int n = v.n, me = v.me, tid = threadIdx.x;
   block_fsub<NT>(v.H, v.hrcp, n, v.u);  // u = L^{-1} p1
   if (me > 0) {
-    for (int c = tid; c < me; c += NT) {
-      double acc = 0.0;
-      for (int i = 0; i < n; ++i) acc += v.W[i * me + c] * v.u[i];
-      v.t[c] = acc - v4[c];
+    if (n >= kMfmaMinN && n <= kMfmaMaxN && me <= 64) {  // t = W^T u - v4 as column sums
+      QPX_DYN_LDS(qpx_smem);
+      double acc[1] = {0.0};
+      colsum_acc<NT, 1>(v.W, n, me, v.u, acc);
+      colsum_finish<NT, 1>(acc, me, v4, -1.0, 1.0, v.t, reinterpret_cast<double*>(qpx_smem));
+    } else {
+      for (int c = tid; c < me; c += NT) {
+        double acc = 0.0;
+        for (int i = 0; i < n; ++i) acc += v.W[i * me + c] * v.u[i];
+        v.t[c] = acc - v4[c];
+      }
+      __syncthreads();
     }
-    __syncthreads();
     block_fsub<NT>(v.F, v.frcp, me, v.t);
     block_bsub_t<NT>(v.F, v.frcp, me, v.t);
     for (int c = tid; c < me; c += NT) v.dy[c] = v.t[c];

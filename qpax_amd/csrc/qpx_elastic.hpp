@@ -122,16 +122,27 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
     __syncthreads();
   }
   double m = 0.0;
-  for (int i = tid; i < n; i += NT) {
-    double acc = v.q[i];
-    if (wide)
-      for (int j = 0; j < n; ++j) acc += v.Q[j * n + i] * v.x[j];
-    else
-      for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
-    for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * e.z2[k];
-    for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
-    v.r1[i] = acc;
-    m = nmax(m, fabs(acc));
+  if (wide && n <= kMfmaMaxN) {  // r1 = q + Q x + G^T z2 + A^T y as column sums (Q is symmetric)
+    constexpr int CPL = colsum_slots<NT>();
+    QPX_DYN_LDS(qpx_smem);
+    double acc[CPL] = {};
+    colsum_acc<NT, CPL>(v.Q, n, n, v.x, acc);
+    colsum_acc<NT, CPL>(G, mi, n, e.z2, acc);
+    colsum_acc<NT, CPL>(v.A, me, n, v.y, acc);
+    colsum_finish<NT, CPL>(acc, n, v.q, 1.0, 1.0, v.r1, reinterpret_cast<double*>(qpx_smem));
+    for (int i = tid; i < n; i += NT) m = nmax(m, fabs(v.r1[i]));
+  } else {
+    for (int i = tid; i < n; i += NT) {
+      double acc = v.q[i];
+      if (wide)
+        for (int j = 0; j < n; ++j) acc += v.Q[j * n + i] * v.x[j];
+      else
+        for (int j = 0; j < n; ++j) acc += v.Q[i * n + j] * v.x[j];
+      for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * e.z2[k];
+      for (int c = 0; c < me; ++c) acc += v.A[c * n + i] * v.y[c];
+      v.r1[i] = acc;
+      m = nmax(m, fabs(acc));
+    }
   }
   for (int k = tid; k < mi; k += NT) {
     double g = 0.0;
@@ -183,10 +194,18 @@ __device__ void e_solve(EView& e, const double* R1, const double* R2, const doub
     e.pa3[k] = p1 / (a1 + a2);
   }
   __syncthreads();
-  for (int i = tid; i < n; i += NT) {
-    double acc = 0.0;
-    for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * e.pa3[k];
-    v.u[i] = sg * R1[i] - acc;
+  if (n >= kMfmaMinN && n <= kMfmaMaxN) {  // u = sg R1 - G^T (p1 / a3) as column sums
+    constexpr int CPL = colsum_slots<NT>();
+    QPX_DYN_LDS(qpx_smem);
+    double acc[CPL] = {};
+    colsum_acc<NT, CPL>(G, mi, n, e.pa3, acc);
+    colsum_finish<NT, CPL>(acc, n, R1, sg, -1.0, v.u, reinterpret_cast<double*>(qpx_smem));
+  } else {
+    for (int i = tid; i < n; i += NT) {
+      double acc = 0.0;
+      for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * e.pa3[k];
+      v.u[i] = sg * R1[i] - acc;
+    }
   }
   for (int c = tid; c < me; c += NT) v.dy[c] = sg * R7[c];
   __syncthreads();

@@ -14,6 +14,7 @@ non-zero exit status.  Case kinds (the npz's `kind`):
 * "uniform": batched arrays Q (B,n,n), ...; runs qpx_solve_qp_relaxed and qpx_diff_qp in `mode`.
 * "init", "init_elastic", "trace": qpx_initialize, qpx_initialize_elastic (with `penalty`) and
   qpx_solve_qp_trace (with `tol`, `debug_step`) on batched arrays.
+* "elastic": qpx_solve_qp_elastic_relaxed and qpx_diff_qp_elastic (with `penalty`, `mode`).
 
 The output npz holds every output array (flat, in the layout of the C ABI).
 """
@@ -215,12 +216,47 @@ def run_trace(e: Emu, c) -> dict:
             "trace": e.host(tr, B * 150, np.float64).reshape(B, 30, 5)}
 
 
+def run_elastic(e: Emu, c) -> dict:
+    """qpx_solve_qp_elastic_relaxed (relaxation `flags`: 0 the reference's, 2 consistent signs) and
+    qpx_diff_qp_elastic with dl/dx = 2 (x - 1) on batched arrays Q, q, A, b, G, h and penalty (B,),
+    in `mode`."""
+    lib = e.lib
+    dt, code, (Q, q, A, b, G, h, pen) = _batch_args(e, c, ("Q", "q", "A", "b", "G", "h", "penalty"))
+    B, n, me, mi = Q.shape[0], Q.shape[1], A.shape[1], G.shape[1]
+    mode = 1 if str(c["mode"]) == "fused" else 0
+    ops = [e.dev(v) for v in (Q, q, A, b, G, h, pen)]
+    wsb = lib.qpx_workspace_size(B, n, me, mi)
+    ws = e.empty(wsb, np.uint8)
+    sizes = [B * n] + [B * mi] * 5 + [B * me]
+    outs = [e.empty(k, dt) for k in sizes]
+    routs = [e.empty(k, dt) for k in sizes]
+    conv, it, rconv, rit = (e.empty(B, np.int32) for _ in range(4))
+    flags = int(c["flags"]) if "flags" in c.files else 0
+    e.check(lib.qpx_solve_qp_elastic_relaxed(code, mode, B, n, me, mi, *ops, float(c["tol"]), float(c["kappa"]), flags,
+                                             *outs, conv, it, *routs, rconv, rit, ws, wsb, None),
+            "solve_qp_elastic_relaxed")
+    xh = e.host(outs[0], B * n, dt)
+    g = e.dev((2.0 * (xh - 1.0)).astype(dt))
+    gsz = (B * n * n, B * n, B * me * n, B * me, B * mi * n, B * mi)
+    grads = [e.empty(k, dt) for k in gsz]
+    e.check(lib.qpx_diff_qp_elastic(code, mode, B, n, me, mi, *ops, *routs, g, 0, *grads, ws, wsb, None),
+            "diff_qp_elastic")
+    names = ("x", "t", "s1", "s2", "z1", "z2", "y")
+    out = {k: e.host(o, sz, dt) for k, o, sz in zip(names, outs, sizes)}
+    out.update({k + "r": e.host(o, sz, dt) for k, o, sz in zip(names, routs, sizes)})
+    out.update({"conv": e.host(conv, B, np.int32), "iters": e.host(it, B, np.int32),
+                "rconv": e.host(rconv, B, np.int32), "riters": e.host(rit, B, np.int32)})
+    for k, name in enumerate(("dQ", "dq", "dA", "db", "dG", "dh")):
+        out[name] = e.host(grads[k], gsz[k], dt)
+    return out
+
+
 def main(argv):
     case = np.load(argv[1], allow_pickle=False)
     e = Emu()
     kind = str(case["kind"])
     out = {"ragged": run_ragged, "uniform": run_uniform, "init": run_init, "init_elastic": run_init_elastic,
-           "trace": run_trace}[kind](e, case)
+           "trace": run_trace, "elastic": run_elastic}[kind](e, case)
     np.savez(argv[2], **out)
     print(f"emu: {kind} case done", flush=True)
 

@@ -250,3 +250,31 @@ def test_emulated_box_rows_with_infinite_bounds(emu, tmp_path):
     fin = np.isfinite(so)
     np.testing.assert_array_equal(np.isinf(out["s"].reshape(B, mi)), ~fin)
     assert np.abs(out["z"].reshape(B, mi)[~fin]).max() == 0.0
+
+
+@pytest.mark.parametrize("n,me,mi", [(70, 3, 90), (100, 0, 130)], ids=["n70-128thr", "n100-256thr"])
+def test_emulated_elastic_wide(emu, tmp_path, n, me, mi):
+    """The elastic kernels at matrix-core sizes (column sums through the G^T D G chunk's LDS, tiled
+    factorisation, row products) in both modes: every access in bounds, forward and gradients
+    against the oracle (the consistent relaxation: the reference's flipped signs amplify rounding
+    differences about 2x per relaxation step)."""
+    if me:
+        Q, q, A, b, G, h, _ = P.elastic_infeasible(1, n, mi, frac=1.0, seed=11, me=me)
+    else:
+        Q, q, G, h, _ = P.elastic_infeasible(1, n, mi, frac=1.0, seed=11)
+        A, b = np.zeros((1, 0, n)), np.zeros((1, 0))
+    pen = np.array([14.0])
+    for mode in ("fused", "plain"):
+        out = _run(emu, dict(kind="elastic", dtype="float64", mode=mode, Q=Q, q=q, A=A, b=b, G=G, h=h, penalty=pen,
+                             tol=1e-6, kappa=1e-3, flags=2), tmp_path, f"el_{mode}")
+        kw = dict(A=A, b=b) if me else {}
+        xo, rel, info = O.solve_qp_elastic_primal_fwd(Q, q, G, h, pen, solver_tol=1e-6, target_kappa=1e-3,
+                                                      consistent=True, **kw)
+        assert out["conv"][0] == info[0][0] == 1
+        assert abs(int(out["iters"][0]) - int(info[1][0])) <= 1
+        assert rel_err(out["x"], xo.ravel()).max() < 1e-8
+        go = O.solve_qp_elastic_primal_bwd(Q, q, G, h, rel, 2 * (xo - 1), **({"A": A} if me else {}))
+        for name, g in zip(("dQ", "dq", "dG", "dh"), go[:4]):
+            e = rel_err(out[name], np.ravel(g)).max()
+            print(f"{mode} {name}: {e:.2e}")
+            assert e < 1e-6, name

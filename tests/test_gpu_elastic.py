@@ -43,7 +43,8 @@ def _elastic_batch(B, n, mi, seed, dt):
 
 
 CASES = [("small-f64", 32, 12, 30, np.float64, 1e-8), ("small-f32", 32, 12, 30, np.float32, 1e-4),
-         ("c5-shape-mini", 4, 64, 128, np.float32, 1e-4)]
+         ("c5-shape-mini", 4, 64, 128, np.float32, 1e-4), ("wide-f64", 4, 100, 200, np.float64, 1e-8),
+         ("n70-f64", 4, 70, 90, np.float64, 1e-8)]
 
 
 @pytest.mark.parametrize("name,B,n,mi,dt,xtol", CASES, ids=[c[0] for c in CASES])
