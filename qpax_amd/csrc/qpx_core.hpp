@@ -277,23 +277,35 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline int gtdg_stride(int n) { return 16 * (((n + 15) / 16) | 1); }
 constexpr int kPanelLd = 17;  // row stride of the Cholesky panel in LDS
+// One Cholesky panel in LDS: a 16-column block column of an n x n matrix (row stride kPanelLd).
+__host__ __device__ inline int panel_words(int n) { return kPanelLd * 16 * ((n + 15) / 16); }
 // Dynamic LDS bytes the generic kernels need for a problem with n variables: the G chunk of the
-// matrix-core G^T D G, and (aliasing it) the 16-column panel of the blocked Cholesky.
+// matrix-core G^T D G, and (aliasing it) the two panels of the blocked Cholesky.
 __host__ __device__ inline int gtdg_lds_bytes(int n) {
   if (n < kMfmaMinN || n > kMfmaMaxN) return 0;
-  const int a = kGtdgRows * gtdg_stride(n) + kGtdgRows, b = kPanelLd * 16 * ((n + 15) / 16);
+  const int a = kGtdgRows * gtdg_stride(n) + kGtdgRows, b = 2 * panel_words(n);
   return 8 * (a > b ? a : b);
 }
-// W = L^{-1} A^T (n x me float64) is formed in LDS, aliasing the G^T D G chunk and the Cholesky
-// panel (both finished by then), when the factorisation is the tiled one and W fits this budget.
+// W = L^{-1} A^T (n x me float64) is formed in LDS when the factorisation is the tiled one, W fits
+// this budget and the workgroup has 256 threads (in the 64/128-thread kernels, which also serve
+// n < 64, the LDS path's registers would lower the occupancy of every launch).  It is formed inside
+// the Cholesky, beside its two panels, when all three fit kFusedLdsMax; otherwise after it,
+// aliasing them.
 constexpr int kWLdsMax = 128 * 1024;
-__host__ __device__ inline bool w_in_lds(int n, int me) {
-  return n >= kMfmaMinN && n <= kMfmaMaxN && me > 0 && 8 * n * me <= kWLdsMax;
+constexpr int kFusedLdsMax = 144 * 1024;
+__host__ __device__ inline bool w_in_lds(int n, int me, int nt) {
+  return nt >= 256 && n >= kMfmaMinN && n <= kMfmaMaxN && me > 0 && 8 * n * me <= kWLdsMax;
 }
-// Dynamic LDS bytes of the generic kernels for a problem of n variables and me equality rows.
-__host__ __device__ inline int generic_lds_bytes(int n, int me) {
-  const int g = gtdg_lds_bytes(n), w = w_in_lds(n, me) ? 8 * n * me : 0;
-  return g > w ? g : w;
+__host__ __device__ inline bool w_fused(int n, int me, int nt) {
+  return w_in_lds(n, me, nt) && 8 * (2 * panel_words(n) + n * me) <= kFusedLdsMax;
+}
+// Dynamic LDS bytes of the nt-thread generic kernels for a problem of n variables and me equality
+// rows.
+__host__ __device__ inline int generic_lds_bytes(int n, int me, int nt) {
+  int b = gtdg_lds_bytes(n);
+  if (w_in_lds(n, me, nt) && 8 * n * me > b) b = 8 * n * me;
+  if (w_fused(n, me, nt) && 8 * (2 * panel_words(n) + n * me) > b) b = 8 * (2 * panel_words(n) + n * me);
+  return b;
 }
 
 // Sum over each group of 16 lanes (DPP row butterflies); every lane of the group receives it.
@@ -553,43 +565,51 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
 }
 
 // Blocked Cholesky for 64 <= m <= kMfmaMaxN, same contract as block_cholesky.  Block column k
-// (16 columns, rows 16k..m-1) is loaded into the LDS panel, factored there column by column, and
-// written back; the trailing tiles (I, J), k < J <= I, then receive -L_Ik L_Jk^T as four
-// v_mfma_f64_16x16x4f64 each, read-modify-written in place, spread round-robin over the
-// wavefronts.
+// (16 columns, rows 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one
+// wavefront's registers, the rows below one per thread, and the panel is written back.  The
+// trailing tiles (I, J), k < J <= I, then receive -L_Ik L_Jk^T as four v_mfma_f64_16x16x4f64 each,
+// two tiles at a time per wavefront with the next pair's loads in flight; the tiles of block
+// column k + 1 go to the second panel buffer, which is the next step's panel, the others back to
+// M.  With Wl (n x me in LDS beside the panels, holding A^T on entry), W = L^{-1} A^T is formed
+// alongside from each panel in the order block_w_lds uses.  P: two panels of panel_words(m).
 template <int NT>
-__device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rcp, double* P) {
+__device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rcp, double* P,
+                                                  double* Wl, int me) {
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
-  const int nt = (m + 15) / 16;
+  const int nt = (m + 15) / 16, ct = (me + 15) / 16;
   const QPX_GLOBAL_AS double* Mg = as_global(M);
+  double* cur = P;
+  double* nxt = P + panel_words(m);
   for (int i = tid; i < m; i += NT) diag0[i] = M[i * m + i];
-  __syncthreads();
-  for (int k = 0; k < nt; ++k) {
-    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16, R = m - c0;
-    // panel: rows c0.., columns c0..c0+cw-1 into P (zero above the diagonal and beyond cw); loads
-    // through clamped indices, eight in flight per thread
-    for (int e0 = 0; e0 < R * 16; e0 += 8 * NT) {
+  // block column 0 from M (zero above the diagonal and beyond 16 columns), eight loads in flight
+  // per thread through clamped indices
+  {
+    const int cw = m < 16 ? m : 16;
+    for (int e0 = 0; e0 < m * 16; e0 += 8 * NT) {
       double val[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int e = e0 + tid + u * NT, r = e >> 4 < R ? e >> 4 : R - 1, c = (e & 15) < cw ? e & 15 : cw - 1;
-        val[u] = Mg[(c0 + r) * m + c0 + c];
+        const int e = e0 + tid + u * NT, r = e >> 4 < m ? e >> 4 : m - 1, c = (e & 15) < cw ? e & 15 : cw - 1;
+        val[u] = Mg[r * m + c];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + tid + u * NT, r = e >> 4, c = e & 15;
-        if (r < R) P[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? val[u] : 0.0;
+        if (r < m) cur[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? val[u] : 0.0;
       }
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int k = 0; k < nt; ++k) {
+    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16, R = m - c0;
     // the 16x16 diagonal block on wavefront 0, lane r holding row r (right-looking, as the scalar
     // block_cholesky: scale column j by 1/L_jj, then subtract its outer product)
     if (w == 0) {
       const int r = l & 15;
       double p[16];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) p[c] = P[r * kPanelLd + c];
+      for (int c = 0; c < 16; ++c) p[c] = cur[r * kPanelLd + c];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         if (j < cw) {
@@ -609,7 +629,7 @@ __device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rc
       }
       if (l < 16)
 #pragma unroll
-        for (int c = 0; c < 16; ++c) P[r * kPanelLd + c] = p[c];
+        for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = p[c];
     }
     __syncthreads();
     // rows below the block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i]) / L_jj,
@@ -617,49 +637,79 @@ __device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rc
     for (int r = 16 + tid; r < R; r += NT) {
       double x[16];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) x[c] = P[r * kPanelLd + c];
+      for (int c = 0; c < 16; ++c) x[c] = cur[r * kPanelLd + c];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         if (j < cw) {
           double vj = x[j];
 #pragma unroll
-          for (int i = 0; i < j; ++i) vj -= x[i] * P[j * kPanelLd + i];
+          for (int i = 0; i < j; ++i) vj -= x[i] * cur[j * kPanelLd + i];
           x[j] = vj * rcp[c0 + j];
         }
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c) P[r * kPanelLd + c] = x[c];
+      for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = x[c];
     }
     __syncthreads();
     for (int e = tid; e < R * 16; e += NT) {
       const int r = e >> 4, c = e & 15;
-      if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = P[r * kPanelLd + c];
+      if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = cur[r * kPanelLd + c];
     }
-    // trailing update of tiles (I, J), k < J <= I < nt: two tiles per wavefront and step, their
-    // loads issued together through clamped indices
-    const int tn = nt - k - 1, T = tn * (tn + 1) / 2;
-    for (int idx0 = w; idx0 < T; idx0 += 2 * W) {
-      int tI[2], tJ[2];
-      d4v acc[2];
+    // W's rows of this block: forward elimination with the diagonal block, lane per column of W
+    if (Wl != nullptr && w == 0) {
+      double rc[16];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int idx = idx0 + t * W < T ? idx0 + t * W : T - 1;
-        int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-        while ((a + 1) * (a + 2) / 2 <= idx) ++a;
-        while (a * (a + 1) / 2 > idx) --a;
-        tI[t] = k + 1 + a;
-        tJ[t] = k + 1 + (idx - a * (a + 1) / 2);
+      for (int i = 0; i < 16; ++i) rc[i] = rcp[c0 + (i < cw ? i : cw - 1)];
+      for (int cg = 0; cg < me; cg += 64) {
+        const int c = cg + l;
+        double wj[16];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
-          acc[t][r] = Mg[(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+        for (int i = 0; i < 16; ++i) {
+          double wi = (i < cw && c < me) ? Wl[(c0 + i) * me + c] : 0.0;
+#pragma unroll
+          for (int j = 0; j < i; ++j) wi -= (i < cw ? cur[i * kPanelLd + j] : 0.0) * wj[j];
+          wi *= i < cw ? rc[i] : 0.0;
+          wj[i] = wi;
+          if (i < cw && c < me) Wl[(c0 + i) * me + c] = wi;
         }
+      }
+    }
+    // trailing update: this wavefront's tiles idx = w, w + W, ... of the T below-diagonal ones,
+    // a pair at a time, the next pair loaded before the current pair's MFMAs
+    const int tn = nt - k - 1, T = tn * (tn + 1) / 2;
+    const int cnt = T > w ? (T - w + W - 1) / W : 0;
+    auto decode = [&](int t, int& I, int& J) {
+      const int idx = w + W * t < T ? w + W * t : T - 1;
+      int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+      while ((a + 1) * (a + 2) / 2 <= idx) ++a;
+      while (a * (a + 1) / 2 > idx) --a;
+      I = k + 1 + a;
+      J = k + 1 + (idx - a * (a + 1) / 2);
+    };
+    auto load = [&](int I, int J, d4v& acc) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
+        acc[r] = Mg[(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+      }
+    };
+    int tI[2] = {0, 0}, tJ[2] = {0, 0}, uI[2] = {0, 0}, uJ[2] = {0, 0};
+    d4v acc[2] = {}, nacc[2] = {};
+    if (cnt > 0) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) { decode(t, tI[t], tJ[t]); load(tI[t], tJ[t], acc[t]); }
+    }
+    const int cw1 = m - c0 - 16 < 16 ? m - c0 - 16 : 16;  // width of block column k + 1
+    for (int pr = 0; pr < cnt; pr += 2) {
+      if (pr + 2 < cnt) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) { decode(pr + 2 + t, uI[t], uJ[t]); load(uI[t], uJ[t], nacc[t]); }
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int ia = 16 * (tI[t] - k) + cq, ib = 16 * (tJ[t] - k) + cq;
-        const double* pa = P + ia * kPanelLd + rq;
-        const double* pb = P + ib * kPanelLd + rq;
+        const double* pa = cur + ia * kPanelLd + rq;
+        const double* pb = cur + ib * kPanelLd + rq;
 #pragma unroll
         for (int sl = 0; sl < 4; ++sl) {
           const double pav = pa[4 * sl], pbv = pb[4 * sl];
@@ -669,15 +719,62 @@ __device__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rc
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        if (idx0 + t * W >= T) continue;
+        if (pr + t >= cnt) continue;
+        if (tJ[t] == k + 1) {  // block column k + 1: into the next panel (zero above its diagonal)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
-          if (i < m && jj < m) M[i * m + jj] = acc[t][r];
+          for (int r = 0; r < 4; ++r) {
+            const int pr_ = 16 * (tI[t] - k - 1) + 4 * r + rq;
+            if (16 * tI[t] + 4 * r + rq < m)
+              nxt[pr_ * kPanelLd + cq] = (cq < cw1 && (pr_ >= 16 || cq <= pr_)) ? acc[t][r] : 0.0;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
+            if (i < m && jj < m) M[i * m + jj] = acc[t][r];
+          }
         }
       }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) { acc[t] = nacc[t]; tI[t] = uI[t]; tJ[t] = uJ[t]; }
     }
     __syncthreads();
+    if (Wl != nullptr) {
+      // W's rows below the block receive -L[rows, block] W[block, :] as 16x16 tiles (block_w_lds)
+      const int Tw = (nt - k - 1) * ct;
+      for (int idx = w; idx < Tw; idx += W) {
+        const int Rb = k + 1 + idx / ct, C = idx - (idx / ct) * ct;
+        const int ia = 16 * Rb + cq, jb = 16 * C + cq;
+        const int pa = (ia < m ? ia : m - 1) - c0, jbc = jb < me ? jb : me - 1;
+        double av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int kk = 4 * q + rq;
+          const int kc = kk < cw ? kk : cw - 1;
+          const double lv = cur[pa * kPanelLd + kc], wv = Wl[(c0 + kc) * me + jbc];
+          av[q] = (ia < m && kk < cw) ? -lv : 0.0;
+          bv[q] = (kk < cw && jb < me) ? wv : 0.0;
+        }
+        d4v wacc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * Rb + 4 * r + rq;
+          const double wv = Wl[(i < m ? i : m - 1) * me + jbc];
+          wacc[r] = (i < m && jb < me) ? wv : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], wacc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * Rb + 4 * r + rq;
+          if (i < m && jb < me) Wl[i * me + jb] = wacc[r];
+        }
+      }
+      __syncthreads();
+    }
+    double* t_ = cur;
+    cur = nxt;
+    nxt = t_;
   }
 }
 
@@ -986,16 +1083,34 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
   }
   __syncthreads();
   const bool tiled = n >= kMfmaMinN && n <= kMfmaMaxN;
+  const bool fused_w = me > 0 && w_fused(n, me, NT);
   if (tiled) {
     QPX_DYN_LDS(qpx_smem);
-    block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, reinterpret_cast<double*>(qpx_smem));
+    double* lds = reinterpret_cast<double*>(qpx_smem);
+    double* Wl = fused_w ? lds + 2 * panel_words(n) : nullptr;
+    // W starts as A^T beside the panels (past the finished G^T D G chunk)
+    if (fused_w)
+      for (int e = tid; e < n * me; e += NT) {
+        const int i = e / me, c = e - i * me;
+        Wl[e] = v.A[c * n + i];
+      }
+    __syncthreads();
+    block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, lds, Wl, me);
   } else {
     block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
   }
   if (me == 0) return;
-  // (256-thread kernels only: in the 64/128-thread ones, which also serve n < 64, the LDS path's
-  // registers would lower the occupancy of every launch)
-  if (NT >= 256 && w_in_lds(n, me)) {
+  if (fused_w) {
+    // F = W^T W on the matrix cores; W is copied to the workspace for the solves
+    QPX_DYN_LDS(qpx_smem);
+    const double* Wl = reinterpret_cast<const double*>(qpx_smem) + 2 * panel_words(n);
+    wtw_mfma<NT>(Wl, n, me, v.F);
+    for (int e = tid; e < n * me; e += NT) v.W[e] = Wl[e];
+    __syncthreads();
+    block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
+    return;
+  }
+  if (w_in_lds(n, me, NT)) {
     // W in LDS (aliasing the finished G^T D G chunk and Cholesky panel), its trailing updates and
     // F = W^T W on the matrix cores; W is copied to the workspace for the solves
     QPX_DYN_LDS(qpx_smem);

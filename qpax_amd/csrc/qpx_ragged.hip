@@ -365,7 +365,7 @@ QPX_API qpx_status qpx_plan_create(int32_t dtype, int64_t batch, const int32_t* 
       total += cost[k];
       if (bks[j].kind == kWave) lds = std::max(lds, qpx::wave::lds_layout(e.n, e.me, e.mi, ts).bytes);
       if (bks[j].kind == kGroup) lds = std::max(lds, qpx::host::group_lds_bytes(e.n, e.me, e.mi, ts, bks[j].gv));
-      if (bks[j].kind == kBlock) lds = std::max(lds, qpx::generic_lds_bytes(e.n, e.me));
+      if (bks[j].kind == kBlock) lds = std::max(lds, qpx::generic_lds_bytes(e.n, e.me, bks[j].nt));
     }
     std::vector<size_t> idx(mem.size());
     std::iota(idx.begin(), idx.end(), 0);

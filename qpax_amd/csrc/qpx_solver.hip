@@ -788,7 +788,7 @@ bool wave_plan(qpx::Args<TS>& a, qpx::host::WaveVariant* v) {
 
 template <int NT, typename TS>
 qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
-  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me, NT);  // G^T D G staging, panels, W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -849,7 +849,7 @@ qpx_status run_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
 
 template <int NT, typename TS>
 qpx_status run_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
-  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me, NT);  // G^T D G staging, panels, W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -922,7 +922,7 @@ qpx_status backward_impl(int mode, int64_t B, int n, int me, int mi, const void*
 // ------------------------------------ elastic -----------------------------------------------
 template <int NT, typename TS>
 qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st) {
-  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me, NT);  // G^T D G staging, panels, W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -959,7 +959,7 @@ qpx_status run_e_forward(qpx::Args<TS>& a, int mode, bool relax, hipStream_t st)
 
 template <int NT, typename TS>
 qpx_status run_e_backward(qpx::Args<TS>& a, int mode, hipStream_t st) {
-  const int lds = qpx::generic_lds_bytes(a.n, a.me);  // G^T D G staging and W (n >= 64)
+  const int lds = qpx::generic_lds_bytes(a.n, a.me, NT);  // G^T D G staging, panels, W (n >= 64)
   const dim3 grid((unsigned)a.B), block(NT);
   if (a.B == 0) return QPX_OK;
   if (mode == QPX_FUSED) {
@@ -1278,8 +1278,8 @@ QPX_API qpx_status qpx_initialize(int32_t dtype, int64_t batch, int32_t n, int32
     a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
     a.G = (const TS*)G; a.h = (const TS*)h;
     a.x = (TS*)x; a.s = (TS*)s; a.z = (TS*)z; a.y = (TS*)y;
-    const int lds = qpx::generic_lds_bytes(n, m_eq);
     return dispatch_nt(n, m_ineq, [&](auto nt) {
+      const int lds = qpx::generic_lds_bytes(n, m_eq, decltype(nt)::value);
       hipLaunchKernelGGL((qpx::k_initialize<decltype(nt)::value, TS>), dim3((unsigned)batch), dim3(decltype(nt)::value),
                          lds, (hipStream_t)stream, a);
       QPX_HIP(hipGetLastError());
@@ -1307,8 +1307,8 @@ QPX_API qpx_status qpx_initialize_elastic(int32_t dtype, int64_t batch, int32_t 
     a.Q = (const TS*)Q; a.q = (const TS*)q; a.A = (const TS*)A; a.b = (const TS*)b;
     a.G = (const TS*)G; a.h = (const TS*)h; a.pen = (const TS*)penalty;
     a.x = (TS*)x; a.t = (TS*)t; a.s = (TS*)s1; a.s2 = (TS*)s2; a.z = (TS*)z1; a.z2 = (TS*)z2; a.y = (TS*)y;
-    const int lds = qpx::generic_lds_bytes(n, m_eq);
     return dispatch_nt(n, m_ineq, [&](auto nt) {
+      const int lds = qpx::generic_lds_bytes(n, m_eq, decltype(nt)::value);
       hipLaunchKernelGGL((qpx::k_e_initialize<decltype(nt)::value, TS>), dim3((unsigned)batch),
                          dim3(decltype(nt)::value), lds, (hipStream_t)stream, a);
       QPX_HIP(hipGetLastError());
@@ -1338,8 +1338,8 @@ QPX_API qpx_status qpx_solve_qp_trace(int32_t dtype, int64_t batch, int32_t n, i
     a.G = (const TS*)G; a.h = (const TS*)h;
     a.x = (TS*)x; a.s = (TS*)s; a.z = (TS*)z; a.y = (TS*)y; a.conv = converged; a.iters = iters;
     a.tol = solver_tol; a.flags = flags; a.trace = trace;
-    const int lds = qpx::generic_lds_bytes(n, m_eq);
     return dispatch_nt(n, m_ineq, [&](auto nt) {
+      const int lds = qpx::generic_lds_bytes(n, m_eq, decltype(nt)::value);
       hipLaunchKernelGGL((qpx::k_fused_solve<decltype(nt)::value, TS>), dim3((unsigned)batch),
                          dim3(decltype(nt)::value), lds, (hipStream_t)stream, a, 0);
       QPX_HIP(hipGetLastError());

@@ -252,7 +252,8 @@ def test_emulated_box_rows_with_infinite_bounds(emu, tmp_path):
     assert np.abs(out["z"].reshape(B, mi)[~fin]).max() == 0.0
 
 
-@pytest.mark.parametrize("n,me,mi", [(70, 3, 90), (100, 0, 130)], ids=["n70-128thr", "n100-256thr"])
+@pytest.mark.parametrize("n,me,mi", [(70, 3, 90), (100, 0, 130), (100, 5, 130)],
+                         ids=["n70-128thr", "n100-256thr", "n100-256thr-eq"])
 def test_emulated_elastic_wide(emu, tmp_path, n, me, mi):
     """The elastic kernels at matrix-core sizes (column sums through the G^T D G chunk's LDS, tiled
     factorisation, row products) in both modes: every access in bounds, forward and gradients
