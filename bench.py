@@ -50,17 +50,26 @@ def bytes_per_qp(n, me, mi, isz=4):
     return inputs * isz, fwd_out, (n + inputs) * isz
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary (FETCH_SIZE x2 +
-    WRITE_SIZE, calibrated on a plain copy), or None."""
-    path = os.path.join(HERE, "profiles", "k_wave_solve_c3_fp32.md")
+def profiled_traffic(profile, key):
+    """HBM bytes per launch recorded as `traffic_bytes_per_launch <key>: <bytes>` in the committed
+    rocprofv3 summary profiles/<profile> (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a plain
+    copy), or None."""
+    path = os.path.join(HERE, "profiles", profile)
     try:
         for line in open(path):
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            if line.startswith(f"traffic_bytes_per_launch {key}:"):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def hbm_fields(traffic, ms):
+    """Achieved HBM bandwidth of `traffic` bytes moved in `ms` milliseconds, and its fraction of peak."""
+    if traffic is None or ms <= 0:
+        return {"hbm_GBs": None, "hbm_frac": None}
+    gbs = traffic / (ms * 1e-3) / 1e9
+    return {"hbm_GBs": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
 
 
 def make_inputs(torch, dev):
@@ -229,7 +238,11 @@ def uneven_bench(torch, args):
         "buckets": plan.buckets(),
         "roofline": {"kernel": "all buckets of the forward (solve + relax)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_F64_TFLOPS, 4),
+                     "traffic": profiled_traffic("uneven_16384_fp32.md", "forward"),
+                     "traffic_source": "profiles/uneven_16384_fp32.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, "
+                                       "summed over the forward's bucket launches)",
+                     **hbm_fields(profiled_traffic("uneven_16384_fp32.md", "forward"), f_ms),
                      "algorithmic_flops_per_launch": fl},
     }
     if not args.no_cpu:
@@ -280,6 +293,32 @@ def elastic_bench(torch, args):
 
     ms = 1e3 * time_steps(torch, step, args.steps, args.warmup)
     x = step().detach().cpu().numpy().astype(np.float64)
+    if args.profile:
+        print(json.dumps({"profile_steps": args.steps}), flush=True)
+        return
+    # the fused solve + relax launch alone, HIP events on the stream the library launches on
+    from qpax_amd._tensors import normalize
+    from qpax_amd.elastic_qp import _solve
+
+    bt = normalize(*[t.detach() for t in ts])
+    pen_t = torch.full((B,), pen, dtype=torch.float32, device=dev)
+    st = torch.cuda.current_stream()
+    f_ms = []
+    for r in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        outs, conv, iters, routs, rconv, riters = _solve(bt, pen_t, TOL, "fused", relax="reference",
+                                                         target_kappa=KAPPA)
+        e1.record(st)
+        torch.cuda.synchronize()
+        if r:
+            f_ms.append(e0.elapsed_time(e1))
+    f_ms = float(np.mean(f_ms))
+    it_f = iters.cpu().numpy().astype(np.float64)
+    it_r = riters.cpu().numpy().astype(np.float64)
+    fl = sum(flops_per_qp(256, 32, 512, a, b, bwd=False)[0] for a, b in zip(it_f, it_r))
+    achieved = fl / (f_ms * 1e-3) / 1e12
+    traffic = profiled_traffic("elastic_n256_fp32.md", "k_e_fused_solve")
     res = {
         "metric": "fwd+bwd QPs/s, elastic QP n=256 m_eq=32 m_ineq=512",
         "value": round(B / (ms * 1e-3), 2), "unit": "QPs/s", "n_gpus": 1, "steps": args.steps,
@@ -290,6 +329,16 @@ def elastic_bench(torch, args):
                                "solve_qp_elastic_primal fwd+bwd, tol 1e-5, kappa 1e-3",
                    "batch": B, "io_dtype": "f32"},
         "shifted_frac": float(shifted.mean()),
+        "iters_fwd_mean": float(it_f.mean()),
+        "iters_relax_mean": float(it_r.mean()),
+        "kernel_ms": {"fused_solve_relax": round(f_ms, 3)},
+        "roofline": {"kernel": "qpx::k_e_fused_solve<256,float> (solve + relax; float64 MFMA + VALU)",
+                     "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/elastic_n256_fp32.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per "
+                                       "launch)",
+                     **hbm_fields(traffic, f_ms),
+                     "algorithmic_flops_per_launch": fl},
     }
     if not args.no_cpu:
         from oracle import qp_oracle as O
@@ -461,8 +510,9 @@ def main():
             "peak": PEAK_F64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_F64_TFLOPS, 4),
-            "traffic": profiled_traffic("k_wave_solve"),
+            "traffic": profiled_traffic("k_wave_solve_c3_fp32.md", "k_wave_solve"),
             "traffic_source": "profiles/k_wave_solve_c3_fp32.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per launch)",
+            **hbm_fields(profiled_traffic("k_wave_solve_c3_fp32.md", "k_wave_solve"), f_ms),
             "algorithmic_flops_per_launch": fl_fwd,
         },
     }

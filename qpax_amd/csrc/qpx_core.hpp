@@ -303,8 +303,15 @@ __host__ __device__ inline bool w_fused(int n, int me, int nt) {
 // rows.
 __host__ __device__ inline int generic_lds_bytes(int n, int me, int nt) {
   int b = gtdg_lds_bytes(n);
+  // the column sums' partials (colsum_lds_bytes) also live there
+  const int cs = 8 * nt * (nt >= 256 ? (kMfmaMaxN + 63) / 64 : 2);
+  if (n >= kMfmaMinN && n <= kMfmaMaxN && b < cs) b = cs;
   if (w_in_lds(n, me, nt) && 8 * n * me > b) b = 8 * n * me;
-  if (w_fused(n, me, nt) && 8 * (2 * panel_words(n) + n * me) > b) b = 8 * (2 * panel_words(n) + n * me);
+  if (n >= kMfmaMinN && n <= kMfmaMaxN) {
+    // the blocked Cholesky: two panels, W beside them when fused, then 1/L_jj and a column buffer
+    const int c = 8 * (2 * panel_words(n) + (w_fused(n, me, nt) ? n * me : 0) + n + 16);
+    if (c > b) b = c;
+  }
   return b;
 }
 
@@ -474,7 +481,6 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
   const QPX_GLOBAL_AS GT* Gg = as_global(G);
   const int64_t last = (int64_t)mi * n - 1;
   auto fetch = [&](int k0) {
-    if constexpr (!kPf) return;
     const int64_t base = (int64_t)k0 * n;
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
@@ -513,43 +519,52 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
       const int r = e / (ldl - n);
       chunk[r * ldl + n + (e - r * (ldl - n))] = 0.0;
     }
-    fetch(0);
-    for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
-      __syncthreads();
-      if constexpr (kPf) {
+    // (a full batch, the common case, runs its MFMAs unguarded)
+    auto mfmas = [&](auto full) {
+#pragma unroll
+      for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
+        const double* gr = chunk + (4 * ks + rq) * ldl + cq;
+        const double dk = cD[4 * ks + rq];
+        double a[kBatch], bb[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+          a[b] = gr[16 * tI[b]] * dk;
+          bb[b] = gr[16 * tJ[b]];
+        }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b)
+          if (decltype(full)::value || b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
+      }
+    };
+    auto run_mfmas = [&]() {
+      if (nb >= kBatch) mfmas(std::true_type{});
+      else mfmas(std::false_type{});
+    };
+    if constexpr (kPf) {
+      // chunk k0 from the registers into LDS, then chunk k0 + 1's loads in flight under its MFMAs
+      fetch(0);
+      for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
+        __syncthreads();
         const int64_t base = (int64_t)k0 * n;
 #pragma unroll
         for (int u = 0; u < EPT; ++u)
           if (dst[u] >= 0) chunk[dst[u]] = base + tid + NT * u <= last ? (double)reg[u] : 0.0;
         if (tid < kGtdgRows) cD[tid] = k0 + tid < mi ? dreg : 0.0;
-      } else {
+        __syncthreads();
+        if (k0 + kGtdgRows < mi) fetch(k0 + kGtdgRows);
+        run_mfmas();
+      }
+    } else {
+      for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
+        __syncthreads();
         for (int r = 0; r < kGtdgRows; ++r) {
           const int k = k0 + r;
           for (int c = tid; c < n; c += NT) chunk[r * ldl + c] = k < mi ? (double)G[(int64_t)k * n + c] : 0.0;
         }
         for (int r = tid; r < kGtdgRows; r += NT) cD[r] = k0 + r < mi ? D[k0 + r] : 0.0;
+        __syncthreads();
+        run_mfmas();
       }
-      __syncthreads();
-      if (k0 + kGtdgRows < mi) fetch(k0 + kGtdgRows);  // in flight under the MFMAs
-      // (a full batch, the common case, runs its MFMAs unguarded)
-      auto mfmas = [&](auto full) {
-#pragma unroll
-        for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
-          const double* gr = chunk + (4 * ks + rq) * ldl + cq;
-          const double dk = cD[4 * ks + rq];
-          double a[kBatch], bb[kBatch];
-#pragma unroll
-          for (int b = 0; b < kBatch; ++b) {
-            a[b] = gr[16 * tI[b]] * dk;
-            bb[b] = gr[16 * tJ[b]];
-          }
-#pragma unroll
-          for (int b = 0; b < kBatch; ++b)
-            if (decltype(full)::value || b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bb[b], acc[b], 0, 0, 0);
-        }
-      };
-      if (nb >= kBatch) mfmas(std::true_type{});
-      else mfmas(std::false_type{});
     }
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
@@ -564,6 +579,154 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
   __syncthreads();
 }
 
+// Pieces of the blocked Cholesky (64 <= m <= kMfmaMaxN) on block column k, held as a panel in LDS:
+// rows c0..m-1 of columns c0..c0+cw-1 at row stride kPanelLd, zero above the diagonal and beyond cw
+// columns.
+
+// The 16x16 diagonal block on wavefront 0, lane r holding row r (right-looking, as the scalar
+// block_cholesky: scale column j by 1/L_jj, then subtract its outer product); 1/L_jj to rcp and
+// rcpl (LDS).  Per column j, lanes 0..15 publish their unscaled entry of column j in colb (16
+// doubles of LDS) and read the whole column back, so every lane forms L_jj, 1/L_jj and the
+// multipliers a_cj / L_jj itself: one LDS write and read per column, no readlane chain.  The pivot
+// test's reference diagonal is read once per lane for all sixteen columns.
+__device__ __forceinline__ void panel_diag_lds(double* cur, int c0, int cw, const double* diag0, double* rcp,
+                                               double* rcpl, double* colb) {
+  const int l = threadIdx.x & 63, r = l & 15;
+  double p[16], dv[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    p[c] = cur[r * kPanelLd + c];
+    dv[c] = diag0[c0 + (c < cw ? c : cw - 1)];
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (j < cw) {
+      if (l < 16) colb[r] = p[j];
+      __builtin_amdgcn_wave_barrier();
+      double a[16];
+#pragma unroll
+      for (int c = j; c < 16; ++c) a[c] = colb[c];
+      __builtin_amdgcn_wave_barrier();
+      const double d = a[j];
+      const bool ok = d > kPivotRel * fabs(dv[j]);
+      const double ljj = ok ? sqrt(d) : kHugePivot;
+      const double inv = 1.0 / ljj;
+      if (r > j) p[j] *= inv;
+      if (r == j) p[j] = ljj;
+      if (l == 0) {
+        rcp[c0 + j] = inv;
+        rcpl[c0 + j] = inv;
+      }
+#pragma unroll
+      for (int c = j + 1; c < 16; ++c) {
+        const double lcj = a[c] * inv;
+        if (r >= c && c < cw) p[c] -= p[j] * lcj;
+      }
+    }
+  }
+  if (l < 16)
+#pragma unroll
+    for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = p[c];
+}
+
+// The rows below the diagonal block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i])
+// / L_jj, the subtractions in the order of the right-looking scalar factorisation.
+// With skip >= 0, wavefront `skip` takes no rows (it forms W's rows meanwhile).
+template <int NT>
+__device__ __forceinline__ void panel_rows(double* cur, int R, int c0, int cw, const double* rcp, int skip = -1) {
+  const int tid = threadIdx.x, wv = tid >> 6;
+  if (wv == skip) return;
+  const int t = skip >= 0 && wv > skip ? tid - 64 : tid, nth = skip >= 0 ? NT - 64 : NT;
+  double rc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) rc[j] = rcp[c0 + (j < cw ? j : cw - 1)];
+  for (int r = 16 + t; r < R; r += nth) {
+    double x[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x[c] = cur[r * kPanelLd + c];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (j < cw) {
+        double vj = x[j];
+#pragma unroll
+        for (int i = 0; i < j; ++i) vj -= x[i] * cur[j * kPanelLd + i];
+        x[j] = vj * rc[j];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = x[c];
+  }
+}
+
+// The factored panel into rows c0..m-1 of M's block column.
+template <int NT>
+__device__ __forceinline__ void panel_store(double* M, int m, const double* cur, int R, int c0, int cw) {
+  for (int e = threadIdx.x; e < R * 16; e += NT) {
+    const int r = e >> 4, c = e & 15;
+    if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = cur[r * kPanelLd + c];
+  }
+}
+
+// W's rows of this block (W = L^{-1} A^T, n x me in LDS): forward elimination with the diagonal
+// block on wavefront 0, lane per column of W.
+__device__ __forceinline__ void panel_w_rows(double* __restrict__ Wl, const double* __restrict__ cur,
+                                             const double* __restrict__ rcp, int c0, int cw, int me) {
+  const int l = threadIdx.x & 63;
+  double rc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) rc[i] = rcp[c0 + (i < cw ? i : cw - 1)];
+  for (int cg = 0; cg < me; cg += 64) {
+    const int c = cg + l;
+    double wj[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      double wi = (i < cw && c < me) ? Wl[(c0 + i) * me + c] : 0.0;
+#pragma unroll
+      for (int j = 0; j < i; ++j) wi -= (i < cw ? cur[i * kPanelLd + j] : 0.0) * wj[j];
+      wi *= i < cw ? rc[i] : 0.0;
+      wj[i] = wi;
+      if (i < cw && c < me) Wl[(c0 + i) * me + c] = wi;
+    }
+  }
+}
+
+// W's rows below block k receive -L[rows, block] W[block, :] as 16x16 tiles (block_w_lds's order).
+template <int NT>
+__device__ __forceinline__ void panel_w_trailing(double* Wl, const double* cur, int k, int m, int me) {
+  constexpr int W = NT / 64;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const int nt = (m + 15) / 16, ct = (me + 15) / 16, c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
+  const int Tw = (nt - k - 1) * ct;
+  for (int idx = w; idx < Tw; idx += W) {
+    const int Rb = k + 1 + idx / ct, C = idx - (idx / ct) * ct;
+    const int ia = 16 * Rb + cq, jb = 16 * C + cq;
+    const int pa = (ia < m ? ia : m - 1) - c0, jbc = jb < me ? jb : me - 1;
+    double av[4], bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = 4 * q + rq;
+      const int kc = kk < cw ? kk : cw - 1;
+      const double lv = cur[pa * kPanelLd + kc], wv = Wl[(c0 + kc) * me + jbc];
+      av[q] = (ia < m && kk < cw) ? -lv : 0.0;
+      bv[q] = (kk < cw && jb < me) ? wv : 0.0;
+    }
+    d4v wacc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * Rb + 4 * r + rq;
+      const double wv = Wl[(i < m ? i : m - 1) * me + jbc];
+      wacc[r] = (i < m && jb < me) ? wv : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], wacc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * Rb + 4 * r + rq;
+      if (i < m && jb < me) Wl[i * me + jb] = wacc[r];
+    }
+  }
+}
+
 // Blocked Cholesky for 64 <= m <= kMfmaMaxN, same contract as block_cholesky.  Block column k
 // (16 columns, rows 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one
 // wavefront's registers, the rows below one per thread, and the panel is written back.  The
@@ -571,13 +734,15 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
 // two tiles at a time per wavefront with the next pair's loads in flight; the tiles of block
 // column k + 1 go to the second panel buffer, which is the next step's panel, the others back to
 // M.  With Wl (n x me in LDS beside the panels, holding A^T on entry), W = L^{-1} A^T is formed
-// alongside from each panel in the order block_w_lds uses.  P: two panels of panel_words(m).
+// alongside from each panel in the order block_w_lds uses, its rows of each block on wavefront 1
+// while the others form the panel's rows below the diagonal block.  P: two panels of
+// panel_words(m); aux: m + 16 doubles of LDS for 1/L_jj and the diagonal block's column broadcasts.
 template <int NT>
 __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rcp, double* P,
-                                                  double* Wl, int me) {
+                                                  double* Wl, int me, double* aux) {
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
-  const int nt = (m + 15) / 16, ct = (me + 15) / 16;
+  const int nt = (m + 15) / 16;
   const QPX_GLOBAL_AS double* Mg = as_global(M);
   double* cur = P;
   double* nxt = P + panel_words(m);
@@ -603,77 +768,12 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
   __syncthreads();
   for (int k = 0; k < nt; ++k) {
     const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16, R = m - c0;
-    // the 16x16 diagonal block on wavefront 0, lane r holding row r (right-looking, as the scalar
-    // block_cholesky: scale column j by 1/L_jj, then subtract its outer product)
-    if (w == 0) {
-      const int r = l & 15;
-      double p[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) p[c] = cur[r * kPanelLd + c];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j < cw) {
-          const double d = readlane_d(p[j], j);
-          const bool ok = d > kPivotRel * fabs(diag0[c0 + j]);
-          const double ljj = ok ? sqrt(d) : kHugePivot;
-          const double inv = 1.0 / ljj;
-          if (r > j) p[j] *= inv;
-          if (r == j) p[j] = ljj;
-          if (l == 0) rcp[c0 + j] = inv;
-#pragma unroll
-          for (int c = j + 1; c < 16; ++c) {
-            const double lcj = readlane_d(p[j], c);
-            if (r >= c && c < cw) p[c] -= p[j] * lcj;
-          }
-        }
-      }
-      if (l < 16)
-#pragma unroll
-        for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = p[c];
-    }
+    if (w == 0) panel_diag_lds(cur, c0, cw, diag0, rcp, aux, aux + m);
     __syncthreads();
-    // rows below the block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i]) / L_jj,
-    // the subtractions in the order of the right-looking scalar factorisation
-    for (int r = 16 + tid; r < R; r += NT) {
-      double x[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) x[c] = cur[r * kPanelLd + c];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        if (j < cw) {
-          double vj = x[j];
-#pragma unroll
-          for (int i = 0; i < j; ++i) vj -= x[i] * cur[j * kPanelLd + i];
-          x[j] = vj * rcp[c0 + j];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = x[c];
-    }
+    if (Wl != nullptr && w == 1) panel_w_rows(Wl, cur, aux, c0, cw, me);
+    else panel_rows<NT>(cur, R, c0, cw, aux, Wl != nullptr ? 1 : -1);
     __syncthreads();
-    for (int e = tid; e < R * 16; e += NT) {
-      const int r = e >> 4, c = e & 15;
-      if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = cur[r * kPanelLd + c];
-    }
-    // W's rows of this block: forward elimination with the diagonal block, lane per column of W
-    if (Wl != nullptr && w == 0) {
-      double rc[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) rc[i] = rcp[c0 + (i < cw ? i : cw - 1)];
-      for (int cg = 0; cg < me; cg += 64) {
-        const int c = cg + l;
-        double wj[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          double wi = (i < cw && c < me) ? Wl[(c0 + i) * me + c] : 0.0;
-#pragma unroll
-          for (int j = 0; j < i; ++j) wi -= (i < cw ? cur[i * kPanelLd + j] : 0.0) * wj[j];
-          wi *= i < cw ? rc[i] : 0.0;
-          wj[i] = wi;
-          if (i < cw && c < me) Wl[(c0 + i) * me + c] = wi;
-        }
-      }
-    }
+    panel_store<NT>(M, m, cur, R, c0, cw);
     // trailing update: this wavefront's tiles idx = w, w + W, ... of the T below-diagonal ones,
     // a pair at a time, the next pair loaded before the current pair's MFMAs
     const int tn = nt - k - 1, T = tn * (tn + 1) / 2;
@@ -738,40 +838,9 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
 #pragma unroll
       for (int t = 0; t < 2; ++t) { acc[t] = nacc[t]; tI[t] = uI[t]; tJ[t] = uJ[t]; }
     }
+    // W's rows below this block (they read the panel and W's rows of this block, both final)
+    if (Wl != nullptr) panel_w_trailing<NT>(Wl, cur, k, m, me);
     __syncthreads();
-    if (Wl != nullptr) {
-      // W's rows below the block receive -L[rows, block] W[block, :] as 16x16 tiles (block_w_lds)
-      const int Tw = (nt - k - 1) * ct;
-      for (int idx = w; idx < Tw; idx += W) {
-        const int Rb = k + 1 + idx / ct, C = idx - (idx / ct) * ct;
-        const int ia = 16 * Rb + cq, jb = 16 * C + cq;
-        const int pa = (ia < m ? ia : m - 1) - c0, jbc = jb < me ? jb : me - 1;
-        double av[4], bv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int kk = 4 * q + rq;
-          const int kc = kk < cw ? kk : cw - 1;
-          const double lv = cur[pa * kPanelLd + kc], wv = Wl[(c0 + kc) * me + jbc];
-          av[q] = (ia < m && kk < cw) ? -lv : 0.0;
-          bv[q] = (kk < cw && jb < me) ? wv : 0.0;
-        }
-        d4v wacc;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * Rb + 4 * r + rq;
-          const double wv = Wl[(i < m ? i : m - 1) * me + jbc];
-          wacc[r] = (i < m && jb < me) ? wv : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], wacc, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * Rb + 4 * r + rq;
-          if (i < m && jb < me) Wl[i * me + jb] = wacc[r];
-        }
-      }
-      __syncthreads();
-    }
     double* t_ = cur;
     cur = nxt;
     nxt = t_;
@@ -1069,35 +1138,38 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
   if (dmode != kDgiven)
     for (int k = tid; k < mi; k += NT) v.D[k] = dmode == kDunit ? 1.0 : v.z[k] / v.s[k];
   __syncthreads();
-  if (n >= kMfmaMinN && n <= kMfmaMaxN) {
-    QPX_DYN_LDS(qpx_smem);
-    gtdg_mfma<NT>(v.Q, v.D, v.H, n, mi, reinterpret_cast<double*>(qpx_smem), G);
-  } else {
-    for (int e = tid; e < n * n; e += NT) {
-      const int i = e / n, j = e - i * n;
-      if (j > i) continue;
-      double acc = 0.0;
-      for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * (v.D[k] * (double)G[k * n + j]);
-      v.H[e] = v.Q[e] + acc;
-    }
-  }
-  __syncthreads();
   const bool tiled = n >= kMfmaMinN && n <= kMfmaMaxN;
   const bool fused_w = me > 0 && w_fused(n, me, NT);
-  if (tiled) {
-    QPX_DYN_LDS(qpx_smem);
-    double* lds = reinterpret_cast<double*>(qpx_smem);
-    double* Wl = fused_w ? lds + 2 * panel_words(n) : nullptr;
-    // W starts as A^T beside the panels (past the finished G^T D G chunk)
-    if (fused_w)
-      for (int e = tid; e < n * me; e += NT) {
-        const int i = e / me, c = e - i * me;
-        Wl[e] = v.A[c * n + i];
+  {
+    if (tiled) {
+      QPX_DYN_LDS(qpx_smem);
+      gtdg_mfma<NT>(v.Q, v.D, v.H, n, mi, reinterpret_cast<double*>(qpx_smem), G);
+    } else {
+      for (int e = tid; e < n * n; e += NT) {
+        const int i = e / n, j = e - i * n;
+        if (j > i) continue;
+        double acc = 0.0;
+        for (int k = 0; k < mi; ++k) acc += (double)G[k * n + i] * (v.D[k] * (double)G[k * n + j]);
+        v.H[e] = v.Q[e] + acc;
       }
+    }
     __syncthreads();
-    block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, lds, Wl, me);
-  } else {
-    block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
+    if (tiled) {
+      QPX_DYN_LDS(qpx_smem);
+      double* lds = reinterpret_cast<double*>(qpx_smem);
+      double* Wl = fused_w ? lds + 2 * panel_words(n) : nullptr;
+      // W starts as A^T beside the panels (past the finished G^T D G chunk)
+      if (fused_w)
+        for (int e = tid; e < n * me; e += NT) {
+          const int i = e / me, c = e - i * me;
+          Wl[e] = v.A[c * n + i];
+        }
+      __syncthreads();
+      block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, lds, Wl, me,
+                               lds + 2 * panel_words(n) + (fused_w ? n * me : 0));
+    } else {
+      block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);
+    }
   }
   if (me == 0) return;
   if (fused_w) {
