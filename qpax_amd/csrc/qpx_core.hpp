@@ -37,10 +37,28 @@ __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a 
 #ifndef QPX_GLOBAL_AS
 #define QPX_GLOBAL_AS __attribute__((address_space(1)))
 #endif
+#ifndef QPX_LDS_AS
+#define QPX_LDS_AS __attribute__((address_space(3)))
+#endif
 template <typename T>
 __device__ __forceinline__ const QPX_GLOBAL_AS T* as_global(const T* p) {
   return (const QPX_GLOBAL_AS T*)p;
 }
+// The same for stores (global stores count in vmcnt only; flat stores also hold up LDS waits).
+template <typename T>
+__device__ __forceinline__ QPX_GLOBAL_AS T* as_global_w(T* p) {
+  return (QPX_GLOBAL_AS T*)p;
+}
+// A generic pointer parameter known to address LDS: with this assumption the compiler issues LDS
+// instructions through it instead of flat ones, which count against both the LDS and the
+// vector-memory wait counters.
+#ifndef QPX_ASSUME_LDS
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QPX_ASSUME_LDS(p) __builtin_assume(__builtin_amdgcn_is_shared((const void*)(p)))
+#else
+#define QPX_ASSUME_LDS(p) ((void)0)  // the host pass of the same source
+#endif
+#endif
 // Lane `lane`'s value of v (a uniform result).
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -572,7 +590,157 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * tI[b] + 4 * r + rq, j = 16 * tJ[b] + cq;
-        if (i < n && j <= i) H[i * n + j] = acc[b][r];
+        if (i < n && j <= i) as_global_w(H)[i * n + j] = acc[b][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// G^T D G for the 256-thread blocks with each wavefront's tiles grouped into 4x4 super-blocks of
+// tiles, so that a k-step reads 8 operands from LDS instead of two per tile.  Work units: a full
+// super-block (P, Q), P > Q (16 tiles: rows 4P..4P+3 times columns 4Q..4Q+3, A operands the four
+// row blocks scaled by D, B operands the four column blocks), or a pair of diagonal super-blocks
+// (P1, P1), (P2, P2) (their 10 lower tiles each; a diagonal super-block's four G column blocks
+// serve as both operands).  Wavefront w takes units w, w + 4, ... (one pass over G per four
+// units: two at n = 256).  Every tile accumulates exactly as in gtdg_mfma (from Q's tile, chunk by
+// chunk, k-step by k-step), so H is the same bit for bit.
+constexpr int kSbSlots = 20;
+__device__ __forceinline__ constexpr int sb_tri_i(int s) { return s < 1 ? 0 : s < 3 ? 1 : s < 6 ? 2 : 3; }
+__device__ __forceinline__ constexpr int sb_tri_j(int s) { return s - sb_tri_i(s) * (sb_tri_i(s) + 1) / 2; }
+
+template <typename GT>
+__device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H, int n, int mi, double* chunk,
+                                     const GT* G) {
+  constexpr int NT = 256, W = NT / 64, S = kSbSlots;
+  constexpr int EPT = (kGtdgRows * (16 * ((kMfmaMaxN / 16) | 1)) + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const int nt = (n + 15) / 16, ldl = gtdg_stride(n), ns = (nt + 3) / 4;
+  const int nF = ns * (ns - 1) / 2, U = nF + (ns + 1) / 2;
+  const int csz = kGtdgRows * n;
+  double* cD = chunk + kGtdgRows * ldl;
+  int dst[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = tid + NT * u;
+    dst[u] = e < csz ? (e / n) * ldl + (e - (e / n) * n) : -1;
+  }
+  GT reg[EPT];
+  double dreg = 0.0;
+  const QPX_GLOBAL_AS GT* Gg = as_global(G);
+  const int64_t last = (int64_t)mi * n - 1;
+  auto fetch = [&](int k0) {
+    const int64_t base = (int64_t)k0 * n;
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int64_t g = base + tid + NT * u;
+      reg[u] = Gg[g < last ? g : last];
+    }
+    const int kd = k0 + (tid & (kGtdgRows - 1));
+    dreg = as_global(D)[kd < mi ? kd : mi - 1];
+  };
+  const QPX_GLOBAL_AS double* Qg = as_global(Q);
+  for (int u0 = 0; u0 < U; u0 += W) {
+    const int u = u0 + w;
+    const bool full = u < nF;  // a full super-block, else a pair of diagonal ones (or nothing)
+    int P = 0, Qb = 0;
+    if (full) {
+      P = (int)((1.0 + sqrt(1.0 + 8.0 * u)) * 0.5);
+      while (P * (P - 1) / 2 > u) --P;
+      while ((P + 1) * P / 2 <= u) ++P;
+      Qb = u - P * (P - 1) / 2;
+    } else {
+      P = 2 * (u - nF);  // diagonal super-blocks P and P + 1 (Qb unused)
+    }
+    const bool has = u < U;
+    int tI[S], tJ[S];
+    bool ok[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      int I, J;
+      if (full) {
+        I = 4 * P + (s & 15) / 4;
+        J = 4 * Qb + (s & 3);
+      } else {
+        const int sb = s < 10 ? P : P + 1, t = s < 10 ? s : s - 10;
+        I = 4 * sb + sb_tri_i(t);
+        J = 4 * sb + sb_tri_j(t);
+      }
+      ok[s] = has && (full ? s < 16 : true) && I < nt && J < nt;
+      tI[s] = ok[s] ? I : 0;
+      tJ[s] = ok[s] ? J : 0;
+    }
+    d4v acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * tI[s] + 4 * r + rq, j = 16 * tJ[s] + cq;
+        acc[s][r] = Qg[(i < n ? i : n - 1) * n + (j < n ? j : n - 1)];
+      }
+    __syncthreads();
+    for (int e = tid; e < kGtdgRows * (ldl - n); e += NT) {
+      const int r = e / (ldl - n);
+      chunk[r * ldl + n + (e - r * (ldl - n))] = 0.0;
+    }
+    fetch(0);
+    for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
+      __syncthreads();
+      {
+        const int64_t base = (int64_t)k0 * n;
+#pragma unroll
+        for (int u = 0; u < EPT; ++u)
+          if (dst[u] >= 0) chunk[dst[u]] = base + tid + NT * u <= last ? (double)reg[u] : 0.0;
+        if (tid < kGtdgRows) cD[tid] = k0 + tid < mi ? dreg : 0.0;
+      }
+      __syncthreads();
+      if (k0 + kGtdgRows < mi) fetch(k0 + kGtdgRows);
+      if (!has) continue;
+      if (full) {
+#pragma unroll
+        for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
+          const double* gr = chunk + (4 * ks + rq) * ldl + cq;
+          const double* ga = gr + 64 * P;
+          const double* gb = gr + 64 * Qb;
+          const double dk = cD[4 * ks + rq];
+          double a[4], b[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            a[i] = ga[16 * i] * dk;
+            b[i] = gb[16 * i];
+          }
+#pragma unroll
+          for (int s = 0; s < 16; ++s)
+            acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s / 4], b[s & 3], acc[s], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
+          const double* gr = chunk + (4 * ks + rq) * ldl + cq;
+          const double* g1 = gr + 64 * P;
+          const double dk = cD[4 * ks + rq];
+          double b1[8], a1[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            b1[i] = g1[16 * i];  // column blocks 4P .. 4P + 7: both diagonal super-blocks
+            a1[i] = b1[i] * dk;
+          }
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const int o = s < 10 ? 0 : 4, t = s < 10 ? s : s - 10;
+            acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[o + sb_tri_i(t)], b1[o + sb_tri_j(t)], acc[s], 0, 0, 0);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!ok[s]) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * tI[s] + 4 * r + rq, j = 16 * tJ[s] + cq;
+        if (i < n && j <= i) as_global_w(H)[i * n + j] = acc[s][r];
       }
     }
   }
@@ -589,8 +757,13 @@ __device__ __noinline__ void gtdg_mfma(const double* Q, const double* D, double*
 // doubles of LDS) and read the whole column back, so every lane forms L_jj, 1/L_jj and the
 // multipliers a_cj / L_jj itself: one LDS write and read per column, no readlane chain.  The pivot
 // test's reference diagonal is read once per lane for all sixteen columns.
-__device__ __forceinline__ void panel_diag_lds(double* cur, int c0, int cw, const double* diag0, double* rcp,
-                                               double* rcpl, double* colb) {
+__device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, const double* diag0_, double* rcp_,
+                                               double* rcpl_, double* colb_) {
+  QPX_LDS_AS double* cur = (QPX_LDS_AS double*)cur_;
+  QPX_LDS_AS double* rcpl = (QPX_LDS_AS double*)rcpl_;
+  QPX_LDS_AS double* colb = (QPX_LDS_AS double*)colb_;
+  const QPX_GLOBAL_AS double* diag0 = as_global(diag0_);
+  QPX_GLOBAL_AS double* rcp = as_global_w(rcp_);
   const int l = threadIdx.x & 63, r = l & 15;
   double p[16], dv[16];
 #pragma unroll
@@ -631,9 +804,12 @@ __device__ __forceinline__ void panel_diag_lds(double* cur, int c0, int cw, cons
 
 // The rows below the diagonal block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i])
 // / L_jj, the subtractions in the order of the right-looking scalar factorisation.
-// With skip >= 0, wavefront `skip` takes no rows (it forms W's rows meanwhile).
+// rcp: the LDS copy of 1/L_jj.  With skip >= 0, wavefront `skip` takes no rows (it forms W's rows
+// meanwhile).
 template <int NT>
-__device__ __forceinline__ void panel_rows(double* cur, int R, int c0, int cw, const double* rcp, int skip = -1) {
+__device__ __forceinline__ void panel_rows(double* cur_, int R, int c0, int cw, const double* rcp_, int skip = -1) {
+  QPX_LDS_AS double* cur = (QPX_LDS_AS double*)cur_;
+  const QPX_LDS_AS double* rcp = (const QPX_LDS_AS double*)rcp_;
   const int tid = threadIdx.x, wv = tid >> 6;
   if (wv == skip) return;
   const int t = skip >= 0 && wv > skip ? tid - 64 : tid, nth = skip >= 0 ? NT - 64 : NT;
@@ -660,8 +836,10 @@ __device__ __forceinline__ void panel_rows(double* cur, int R, int c0, int cw, c
 
 // The factored panel into rows c0..m-1 of M's block column.
 template <int NT>
-__device__ __forceinline__ void panel_store(double* M, int m, const double* cur, int R, int c0, int cw) {
-  for (int e = threadIdx.x; e < R * 16; e += NT) {
+__device__ __forceinline__ void panel_store(double* M_, int m, const double* cur_, int R, int c0, int cw, int t0 = 0) {
+  QPX_GLOBAL_AS double* M = as_global_w(M_);
+  const QPX_LDS_AS double* cur = (const QPX_LDS_AS double*)cur_;
+  for (int e = (int)threadIdx.x - t0; e < R * 16; e += NT - t0) {
     const int r = e >> 4, c = e & 15;
     if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = cur[r * kPanelLd + c];
   }
@@ -669,8 +847,11 @@ __device__ __forceinline__ void panel_store(double* M, int m, const double* cur,
 
 // W's rows of this block (W = L^{-1} A^T, n x me in LDS): forward elimination with the diagonal
 // block on wavefront 0, lane per column of W.
-__device__ __forceinline__ void panel_w_rows(double* __restrict__ Wl, const double* __restrict__ cur,
-                                             const double* __restrict__ rcp, int c0, int cw, int me) {
+__device__ __forceinline__ void panel_w_rows(double* Wl_, const double* cur_, const double* rcp_, int c0, int cw,
+                                             int me) {
+  QPX_LDS_AS double* __restrict__ Wl = (QPX_LDS_AS double*)Wl_;
+  const QPX_LDS_AS double* __restrict__ cur = (const QPX_LDS_AS double*)cur_;
+  const QPX_LDS_AS double* __restrict__ rcp = (const QPX_LDS_AS double*)rcp_;
   const int l = threadIdx.x & 63;
   double rc[16];
 #pragma unroll
@@ -690,14 +871,17 @@ __device__ __forceinline__ void panel_w_rows(double* __restrict__ Wl, const doub
   }
 }
 
-// W's rows below block k receive -L[rows, block] W[block, :] as 16x16 tiles (block_w_lds's order).
+// W's rows below block k receive -L[rows, block] W[block, :] as 16x16 tiles (block_w_lds's order),
+// over wavefronts w0..W-1.
 template <int NT>
-__device__ __forceinline__ void panel_w_trailing(double* Wl, const double* cur, int k, int m, int me) {
+__device__ __forceinline__ void panel_w_trailing(double* Wl_, const double* cur_, int k, int m, int me, int w0 = 0) {
+  QPX_LDS_AS double* Wl = (QPX_LDS_AS double*)Wl_;
+  const QPX_LDS_AS double* cur = (const QPX_LDS_AS double*)cur_;
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
   const int nt = (m + 15) / 16, ct = (me + 15) / 16, c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
   const int Tw = (nt - k - 1) * ct;
-  for (int idx = w; idx < Tw; idx += W) {
+  for (int idx = w - w0; idx < Tw; idx += W - w0) {
     const int Rb = k + 1 + idx / ct, C = idx - (idx / ct) * ct;
     const int ia = 16 * Rb + cq, jb = 16 * C + cq;
     const int pa = (ia < m ? ia : m - 1) - c0, jbc = jb < me ? jb : me - 1;
@@ -731,119 +915,199 @@ __device__ __forceinline__ void panel_w_trailing(double* Wl, const double* cur, 
 // (16 columns, rows 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one
 // wavefront's registers, the rows below one per thread, and the panel is written back.  The
 // trailing tiles (I, J), k < J <= I, then receive -L_Ik L_Jk^T as four v_mfma_f64_16x16x4f64 each,
-// two tiles at a time per wavefront with the next pair's loads in flight; the tiles of block
+// kTrailGroup tiles at a time per wavefront with the next group's loads in flight; the tiles of block
 // column k + 1 go to the second panel buffer, which is the next step's panel, the others back to
 // M.  With Wl (n x me in LDS beside the panels, holding A^T on entry), W = L^{-1} A^T is formed
 // alongside from each panel in the order block_w_lds uses, its rows of each block on wavefront 1
 // while the others form the panel's rows below the diagonal block.  P: two panels of
 // panel_words(m); aux: m + 16 doubles of LDS for 1/L_jj and the diagonal block's column broadcasts.
+constexpr int kTrailGroup = 4;  // trailing tiles per group: the next group's loads fly under its MFMAs
 template <int NT>
 __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rcp, double* P,
                                                   double* Wl, int me, double* aux) {
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
   const int nt = (m + 15) / 16;
+  const bool has_w = Wl != nullptr;
+  if (!has_w) Wl = aux;  // never dereferenced then; an LDS address, so the assumption below holds
+  QPX_ASSUME_LDS(P);
+  QPX_ASSUME_LDS(aux);
+  QPX_ASSUME_LDS(Wl);
   const QPX_GLOBAL_AS double* Mg = as_global(M);
+  QPX_GLOBAL_AS double* Mw = as_global_w(M);
   double* cur = P;
   double* nxt = P + panel_words(m);
-  for (int i = tid; i < m; i += NT) diag0[i] = M[i * m + i];
-  // block column 0 from M (zero above the diagonal and beyond 16 columns), eight loads in flight
-  // per thread through clamped indices
-  {
-    const int cw = m < 16 ? m : 16;
-    for (int e0 = 0; e0 < m * 16; e0 += 8 * NT) {
+  for (int i = tid; i < m; i += NT) as_global_w(diag0)[i] = Mg[i * m + i];
+  // rows r0.. of block column kb from M into a panel (zero above the diagonal and beyond the block
+  // column's width), eight loads in flight per thread through clamped indices
+  auto load_panel = [&](double* pnl_, int kb, int r0) {
+    QPX_LDS_AS double* pnl = (QPX_LDS_AS double*)pnl_;
+    const int c0 = 16 * kb, R = m - c0, cw = R < 16 ? R : 16;
+    for (int e0 = 16 * r0; e0 < R * 16; e0 += 8 * NT) {
       double val[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int e = e0 + tid + u * NT, r = e >> 4 < m ? e >> 4 : m - 1, c = (e & 15) < cw ? e & 15 : cw - 1;
-        val[u] = Mg[r * m + c];
+        const int e = e0 + tid + u * NT, r = e >> 4 < R ? e >> 4 : R - 1, c = (e & 15) < cw ? e & 15 : cw - 1;
+        val[u] = Mg[(c0 + r) * m + c0 + c];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + tid + u * NT, r = e >> 4, c = e & 15;
-        if (r < m) cur[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? val[u] : 0.0;
+        if (r < R) pnl[r * kPanelLd + c] = (c < cw && (r >= 16 || c <= r)) ? val[u] : 0.0;
       }
     }
-  }
+  };
+  load_panel(cur, 0, 0);
   __syncthreads();
-  for (int k = 0; k < nt; ++k) {
-    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16, R = m - c0;
-    if (w == 0) panel_diag_lds(cur, c0, cw, diag0, rcp, aux, aux + m);
-    __syncthreads();
-    if (Wl != nullptr && w == 1) panel_w_rows(Wl, cur, aux, c0, cw, me);
-    else panel_rows<NT>(cur, R, c0, cw, aux, Wl != nullptr ? 1 : -1);
-    __syncthreads();
-    panel_store<NT>(M, m, cur, R, c0, cw);
-    // trailing update: this wavefront's tiles idx = w, w + W, ... of the T below-diagonal ones,
-    // a pair at a time, the next pair loaded before the current pair's MFMAs
-    const int tn = nt - k - 1, T = tn * (tn + 1) / 2;
-    const int cnt = T > w ? (T - w + W - 1) / W : 0;
+  if (w == 0) panel_diag_lds(cur, 0, m < 16 ? m : 16, diag0, rcp, aux, aux + m);
+  __syncthreads();
+  auto load = [&](int I, int J, d4v& acc) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
+      acc[r] = Mg[(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+    }
+  };
+  // acc -= L_Ik L_Jk^T from the panel of block column k (rows past m count as zero)
+  auto update = [&](const double* pnl, int k, int R, int I, int J, d4v& acc) {
+    const int ia = 16 * (I - k) + cq, ib = 16 * (J - k) + cq;
+    const double* pa = pnl + ia * kPanelLd + rq;
+    const double* pb = pnl + ib * kPanelLd + rq;
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl) {
+      const double pav = pa[4 * sl], pbv = pb[4 * sl];
+      const double av = ia < R ? -pav : 0.0, bv = ib < R ? pbv : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+  };
+  // the trailing tiles (first + a, first + b), b <= a < tn2, idx >= i0 in row-major order, receive
+  // the updates of block column k (panel pk) and, with pk1, of block column k + 1 (panel pk1): this
+  // wavefront's tiles idx = i0 + wi, i0 + wi + ws, ..., kTrailGroup at a time, the next group
+  // loaded before the current group's MFMAs
+  auto trail = [&](const double* pk, const double* pk1, int k, int first, int tn2, int i0, int wi, int ws) {
+    const int T = tn2 * (tn2 + 1) / 2, R = m - 16 * k;
+    const int cnt = T > i0 + wi ? (T - i0 - wi + ws - 1) / ws : 0;
     auto decode = [&](int t, int& I, int& J) {
-      const int idx = w + W * t < T ? w + W * t : T - 1;
+      const int idx = i0 + wi + ws * t < T ? i0 + wi + ws * t : T - 1;
       int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
       while ((a + 1) * (a + 2) / 2 <= idx) ++a;
       while (a * (a + 1) / 2 > idx) --a;
-      I = k + 1 + a;
-      J = k + 1 + (idx - a * (a + 1) / 2);
+      I = first + a;
+      J = first + (idx - a * (a + 1) / 2);
     };
-    auto load = [&](int I, int J, d4v& acc) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
-        acc[r] = Mg[(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
-      }
-    };
-    int tI[2] = {0, 0}, tJ[2] = {0, 0}, uI[2] = {0, 0}, uJ[2] = {0, 0};
-    d4v acc[2] = {}, nacc[2] = {};
+    constexpr int GS = kTrailGroup;
+    int tI[GS] = {}, tJ[GS] = {}, uI[GS] = {}, uJ[GS] = {};
+    d4v acc[GS] = {}, nacc[GS] = {};
     if (cnt > 0) {
 #pragma unroll
-      for (int t = 0; t < 2; ++t) { decode(t, tI[t], tJ[t]); load(tI[t], tJ[t], acc[t]); }
+      for (int t = 0; t < GS; ++t) { decode(t, tI[t], tJ[t]); load(tI[t], tJ[t], acc[t]); }
     }
-    const int cw1 = m - c0 - 16 < 16 ? m - c0 - 16 : 16;  // width of block column k + 1
-    for (int pr = 0; pr < cnt; pr += 2) {
-      if (pr + 2 < cnt) {
+    for (int pr = 0; pr < cnt; pr += GS) {
+      if (pr + GS < cnt) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) { decode(pr + 2 + t, uI[t], uJ[t]); load(uI[t], uJ[t], nacc[t]); }
+        for (int t = 0; t < GS; ++t) { decode(pr + GS + t, uI[t], uJ[t]); load(uI[t], uJ[t], nacc[t]); }
       }
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int ia = 16 * (tI[t] - k) + cq, ib = 16 * (tJ[t] - k) + cq;
-        const double* pa = cur + ia * kPanelLd + rq;
-        const double* pb = cur + ib * kPanelLd + rq;
+      for (int t = 0; t < GS; ++t) {
+        update(pk, k, R, tI[t], tJ[t], acc[t]);
+        if (pk1 != nullptr) update(pk1, k + 1, R - 16, tI[t], tJ[t], acc[t]);
+      }
+      // every group stores the same number of entries, so that the next group's loads can be
+      // awaited without waiting for these stores: entries outside the matrix or of slots past the
+      // wavefront's tiles go to M[0][m - 1], in the strict upper triangle, which nothing reads
 #pragma unroll
-        for (int sl = 0; sl < 4; ++sl) {
-          const double pav = pa[4 * sl], pbv = pb[4 * sl];
-          const double av = ia < R ? -pav : 0.0, bv = ib < R ? pbv : 0.0;
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+      for (int t = 0; t < GS; ++t) {
+        const bool valid = pr + t < cnt;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
+          Mw[valid && i < m && jj < m ? i * m + jj : m - 1] = acc[t][r];
         }
       }
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        if (pr + t >= cnt) continue;
-        if (tJ[t] == k + 1) {  // block column k + 1: into the next panel (zero above its diagonal)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int pr_ = 16 * (tI[t] - k - 1) + 4 * r + rq;
-            if (16 * tI[t] + 4 * r + rq < m)
-              nxt[pr_ * kPanelLd + cq] = (cq < cw1 && (pr_ >= 16 || cq <= pr_)) ? acc[t][r] : 0.0;
-          }
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
-            if (i < m && jj < m) M[i * m + jj] = acc[t][r];
-          }
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < 2; ++t) { acc[t] = nacc[t]; tI[t] = uI[t]; tJ[t] = uJ[t]; }
+      for (int t = 0; t < GS; ++t) { acc[t] = nacc[t]; tI[t] = uI[t]; tJ[t] = uJ[t]; }
     }
-    // W's rows below this block (they read the panel and W's rows of this block, both final)
-    if (Wl != nullptr) panel_w_trailing<NT>(Wl, cur, k, m, me);
+  };
+  // the first wavefront factors diagonal blocks, the others (all of them with one wavefront) the
+  // rest of each phase
+  const int w0 = W > 1 ? 1 : 0;
+  const bool diag_wave = w == 0, work_wave = W == 1 || w > 0;
+  const int wi = W > 1 ? w - 1 : 0, ws = W > 1 ? W - 1 : 1;
+  // Block columns in pairs (k, k + 1) with A = panel k (diagonal block factored) on entry:
+  //   1. A's rows below the diagonal block (W's rows of block k beside them); store A
+  //   2. block column k + 1 from M with block column k's update into B; its diagonal block factored
+  //      while the others apply block column k to W's rows below it
+  //   3. B's rows below the diagonal block (W's rows of block k + 1); then, while the first
+  //      wavefront forms the diagonal block of block column k + 2 from both panels into A's first
+  //      16 rows (free once A is stored; the trailing update reads A's rows from 32 and B's from
+  //      16) and factors it, the others store B and give every other trailing tile both panels'
+  //      updates in one pass through M, and apply block column k + 1 to W's rows below it
+  //   4. the rows of block column k + 2 below its diagonal block from M into A
+  // Each tile sees the updates of block columns 0, 1, ... in that order, as with one block column
+  // per step, and the trailing matrix goes through M once per pair.
+  double* A = cur;
+  double* B = nxt;
+  auto rows_of = [&](double* pnl, int kb) {
+    const int c0 = 16 * kb, R = m - c0, cw = R < 16 ? R : 16;
+    if (has_w && w == 1) panel_w_rows(Wl, pnl, aux, c0, cw, me);
+    else panel_rows<NT>(pnl, R, c0, cw, aux, has_w ? 1 : -1);
+  };
+  int k = 0;
+  for (; k + 1 < nt; k += 2) {
+    const int c0 = 16 * k, R = m - c0, cw = 16;
+    const int cw1 = R - 16 < 16 ? R - 16 : 16, tn = nt - k - 1;
+    rows_of(A, k);
     __syncthreads();
-    double* t_ = cur;
-    cur = nxt;
-    nxt = t_;
+    panel_store<NT>(M, m, A, R, c0, cw);
+    for (int a = w; a < tn; a += W) {
+      const int I = k + 1 + a;
+      d4v acc;
+      load(I, k + 1, acc);
+      update(A, k, R, I, k + 1, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int pr_ = 16 * a + 4 * r + rq;
+        if (16 * I + 4 * r + rq < m) B[pr_ * kPanelLd + cq] = (cq < cw1 && (pr_ >= 16 || cq <= pr_)) ? acc[r] : 0.0;
+      }
+    }
+    __syncthreads();
+    if (diag_wave) panel_diag_lds(B, c0 + 16, cw1, diag0, rcp, aux, aux + m);
+    if (work_wave && has_w) panel_w_trailing<NT>(Wl, A, k, m, me, w0);
+    __syncthreads();
+    rows_of(B, k + 1);
+    __syncthreads();
+    const bool more = k + 2 < nt;
+    const int cw2 = more ? (R - 32 < 16 ? R - 32 : 16) : 0;
+    if (diag_wave && more) {
+      d4v acc;
+      load(k + 2, k + 2, acc);
+      update(A, k, R, k + 2, k + 2, acc);
+      update(B, k + 1, R - 16, k + 2, k + 2, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int pr_ = 4 * r + rq;
+        if (c0 + 32 + pr_ < m) A[pr_ * kPanelLd + cq] = (cq < cw2 && cq <= pr_) ? acc[r] : 0.0;
+      }
+      __builtin_amdgcn_wave_barrier();
+      panel_diag_lds(A, c0 + 32, cw2, diag0, rcp, aux, aux + m);
+    }
+    if (work_wave) {
+      panel_store<NT>(M, m, B, R - 16, c0 + 16, cw1, 64 * w0);
+      if (more) trail(A, B, k, k + 2, tn - 1, 1, wi, ws);
+      if (has_w) panel_w_trailing<NT>(Wl, B, k + 1, m, me, w0);
+    }
+    __syncthreads();
+    if (more) {
+      load_panel(A, k + 2, 16);
+      __syncthreads();
+    }
+  }
+  if (k < nt) {  // an odd number of block columns: the last one alone
+    const int c0 = 16 * k, R = m - c0, cw = R < 16 ? R : 16;
+    rows_of(A, k);
+    __syncthreads();
+    panel_store<NT>(M, m, A, R, c0, cw);
+    __syncthreads();
   }
 }
 
@@ -943,9 +1207,192 @@ __device__ __forceinline__ void bsub_t_blocks(const double* L, const double* rcp
   }
 }
 
-// The blocked sweeps with b staged in LDS for m <= kMfmaMaxN (the in-block eliminations and the
-// row updates then read and write b without global round trips), in place in b otherwise.
+// Forward substitution L y = b with b in LDS and one barrier per 16-row block: at step k,
+// wavefront 0 gives block k's rows block k - 1's update (y_{k-1} is final) and eliminates inside
+// the block, while the other wavefronts apply block k - 1 to the rows from block k + 1 on (at most
+// two rows per thread: m - 16 <= 2 (NT - 64)).  Each thread's entries of L for step k + 2 are
+// loaded while step k runs: on wavefront 0, lane l's row c0 + l from column c0 - 16 and 1/L_ll;
+// on the others, their two rows' 16 entries of block column k - 1.  Every row sees the blocks'
+// updates in the same order and with the same arithmetic as fsub_blocks.
 template <int NT>
+__device__ __noinline__ void fsub_lds(const double* L, const double* rcp, int m, double* b_) {
+  constexpr int NW = NT - 64;
+  QPX_LDS_AS double* b = (QPX_LDS_AS double*)b_;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, t = tid - 64;
+  const int nt = (m + 15) / 16;
+  const QPX_GLOBAL_AS double* Lg = as_global(L);
+  const QPX_GLOBAL_AS double* rg = as_global(rcp);
+  auto fetch = [&](int k, double (&v)[32], double& rl) {
+    if (k >= nt) return;
+    const int c0 = 16 * k;
+    if (w == 0) {
+      const int cw = m - c0 < 16 ? m - c0 : 16, r = c0 + (l < cw ? l : cw - 1);
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        const int c = c0 - 16 + j, cc = c < 0 ? 0 : (c < m ? c : m - 1);
+        v[j] = Lg[r * m + cc];
+      }
+      rl = rg[r];
+    } else {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = 16 * (k + 1) + t + h * NW, rr = r < m ? r : m - 1;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int c = 16 * (k - 1) + j;
+          v[16 * h + j] = Lg[rr * m + (c < 0 ? 0 : c)];
+        }
+      }
+    }
+  };
+  auto step = [&](int k, double (&v)[32], double& rl) {
+    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
+    if (w == 0) {
+      double bi = l < cw ? b[c0 + l] : 0.0;
+      if (k > 0) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+          a0 += v[j] * b[c0 - 16 + j];
+          a1 += v[j + 1] * b[c0 - 16 + j + 1];
+        }
+        bi -= a0 + a1;
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (j < cw) {
+          const double yj = readlane_d(bi * rl, j);
+          const double lv = (l > j && l < cw) ? v[16 + j] : 0.0;
+          bi = (l == j) ? yj : bi - lv * yj;
+        }
+      }
+      if (l < cw) b[c0 + l] = bi;
+    } else if (k > 0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = 16 * (k + 1) + t + h * NW;
+        if (r < m) {
+          double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+          for (int j = 0; j < 16; j += 2) {
+            a0 += v[16 * h + j] * b[c0 - 16 + j];
+            a1 += v[16 * h + j + 1] * b[c0 - 16 + j + 1];
+          }
+          b[r] -= a0 + a1;
+        }
+      }
+    }
+    fetch(k + 2, v, rl);
+    __syncthreads();
+  };
+  double va[32], vb[32], ra = 0.0, rb = 0.0;
+  fetch(0, va, ra);
+  fetch(1, vb, rb);
+  for (int k = 0; k < nt; k += 2) {
+    step(k, va, ra);
+    if (k + 1 < nt) step(k + 1, vb, rb);
+  }
+}
+
+// Back substitution L^T x = b with b in LDS, blocks from the last, one barrier per block: at step
+// k, wavefront 0 gives block k's rows block k + 1's update and eliminates inside the block (lane l
+// holds column c0 + l of rows c0..c0 + 31), while the other wavefronts apply block k + 1 to the rows
+// above block k.  Same order and arithmetic per row as bsub_t_blocks; m - 16 <= 2 (NT - 64).
+template <int NT>
+__device__ __noinline__ void bsub_t_lds(const double* L, const double* rcp, int m, double* b_) {
+  constexpr int NW = NT - 64;
+  QPX_LDS_AS double* b = (QPX_LDS_AS double*)b_;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, t = tid - 64;
+  const int nt = (m + 15) / 16;
+  const QPX_GLOBAL_AS double* Lg = as_global(L);
+  const QPX_GLOBAL_AS double* rg = as_global(rcp);
+  auto fetch = [&](int k, double (&v)[32], double& rl) {
+    if (k < 0) return;
+    const int c0 = 16 * k;
+    if (w == 0) {
+      const int cw = m - c0 < 16 ? m - c0 : 16, c = c0 + (l < cw ? l : cw - 1);
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        const int r = c0 + j;
+        v[j] = Lg[(r < m ? r : m - 1) * m + c];
+      }
+      rl = rg[c];
+    } else {
+      const int c1 = c0 + 16;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = t + h * NW, rr = r < c0 ? r : 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int i = c1 + j;
+          v[16 * h + j] = Lg[(i < m ? i : m - 1) * m + rr];
+        }
+      }
+    }
+  };
+  auto step = [&](int k, double (&v)[32], double& rl) {
+    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
+    const int c1 = c0 + 16, cw1 = m - c1 < 16 ? m - c1 : 16;  // block k + 1 (when k + 1 < nt)
+    if (w == 0) {
+      double bi = l < cw ? b[c0 + l] : 0.0;
+      if (k + 1 < nt) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+          if (j < cw1) a0 += v[16 + j] * b[c1 + j];
+          if (j + 1 < cw1) a1 += v[16 + j + 1] * b[c1 + j + 1];
+        }
+        bi -= a0 + a1;
+      }
+#pragma unroll
+      for (int j = 15; j >= 0; --j) {
+        if (j < cw) {
+          const double xj = readlane_d(bi * rl, j);
+          const double lv = (l < j) ? v[j] : 0.0;
+          bi = (l == j) ? xj : bi - lv * xj;
+        }
+      }
+      if (l < cw) b[c0 + l] = bi;
+    } else if (k + 1 < nt) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = t + h * NW;
+        if (r < c0) {
+          double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+          for (int j = 0; j < 16; j += 2) {
+            if (j < cw1) a0 += v[16 * h + j] * b[c1 + j];
+            if (j + 1 < cw1) a1 += v[16 * h + j + 1] * b[c1 + j + 1];
+          }
+          b[r] -= a0 + a1;
+        }
+      }
+    }
+    fetch(k - 2, v, rl);
+    __syncthreads();
+  };
+  double va[32], vb[32], ra = 0.0, rb = 0.0;
+  fetch(nt - 1, va, ra);
+  fetch(nt - 2, vb, rb);
+  for (int k = nt - 1; k >= 0; k -= 2) {
+    step(k, va, ra);
+    if (k - 1 >= 0) step(k - 1, vb, rb);
+  }
+}
+
+// the one-barrier sweeps need every row below (above) a block on at most two threads of the
+// wavefronts past the first
+template <int NT>
+__host__ __device__ constexpr bool sweep_lds_ok(int m) {
+  return NT >= 128 && m - 16 <= 2 * (NT - 64);
+}
+
+// The blocked sweeps with b staged in LDS for m <= kMfmaMaxN (the in-block eliminations and the
+// row updates then read and write b without global round trips), in place in b otherwise.  PF
+// selects the one-barrier sweeps with L loaded two blocks ahead (fsub_lds / bsub_t_lds): their
+// 248 VGPRs would cap every kernel that links them at one wavefront per SIMD, so only kernels that
+// run at that occupancy anyway (the elastic kernels: one 256-thread workgroup per CU) take them.
+template <int NT, bool PF = false>
 __device__ void block_fsub_tiled(const double* L, const double* rcp, int m, double* b) {
   __shared__ double sb[kMfmaMaxN];
   if (m > kMfmaMaxN) {
@@ -954,12 +1401,13 @@ __device__ void block_fsub_tiled(const double* L, const double* rcp, int m, doub
   }
   for (int i = threadIdx.x; i < m; i += NT) sb[i] = b[i];
   __syncthreads();
-  fsub_blocks<NT>(L, rcp, m, sb);
+  if (PF && sweep_lds_ok<NT>(m)) fsub_lds<NT>(L, rcp, m, sb);
+  else fsub_blocks<NT>(L, rcp, m, sb);
   for (int i = threadIdx.x; i < m; i += NT) b[i] = sb[i];
   __syncthreads();
 }
 
-template <int NT>
+template <int NT, bool PF = false>
 __device__ void block_bsub_t_tiled(const double* L, const double* rcp, int m, double* b) {
   __shared__ double sb[kMfmaMaxN];
   if (m > kMfmaMaxN) {
@@ -968,7 +1416,8 @@ __device__ void block_bsub_t_tiled(const double* L, const double* rcp, int m, do
   }
   for (int i = threadIdx.x; i < m; i += NT) sb[i] = b[i];
   __syncthreads();
-  bsub_t_blocks<NT>(L, rcp, m, sb);
+  if (PF && sweep_lds_ok<NT>(m)) bsub_t_lds<NT>(L, rcp, m, sb);
+  else bsub_t_blocks<NT>(L, rcp, m, sb);
   for (int i = threadIdx.x; i < m; i += NT) b[i] = sb[i];
   __syncthreads();
 }
@@ -1120,15 +1569,15 @@ __device__ void wtw_mfma(const double* Wl, int n, int me, double* F) {
 }
 
 // Triangular solves with a factor from block_cholesky[_tiled]: blocked from m = 64 on.
-template <int NT>
+template <int NT, bool PF = false>
 __device__ void block_fsub(const double* L, const double* rcp, int m, double* b) {
-  if (m >= kMfmaMinN) block_fsub_tiled<NT>(L, rcp, m, b);
+  if (m >= kMfmaMinN) block_fsub_tiled<NT, PF>(L, rcp, m, b);
   else block_fsub_scalar<NT>(L, rcp, m, b);
 }
 
-template <int NT>
+template <int NT, bool PF = false>
 __device__ void block_bsub_t(const double* L, const double* rcp, int m, double* b) {
-  if (m >= kMfmaMinN) block_bsub_t_tiled<NT>(L, rcp, m, b);
+  if (m >= kMfmaMinN) block_bsub_t_tiled<NT, PF>(L, rcp, m, b);
   else block_bsub_t_scalar<NT>(L, rcp, m, b);
 }
 
@@ -1143,7 +1592,8 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
   {
     if (tiled) {
       QPX_DYN_LDS(qpx_smem);
-      gtdg_mfma<NT>(v.Q, v.D, v.H, n, mi, reinterpret_cast<double*>(qpx_smem), G);
+      if constexpr (NT == 256) gtdg_sb<GT>(v.Q, v.D, v.H, n, mi, reinterpret_cast<double*>(qpx_smem), G);
+      else gtdg_mfma<NT>(v.Q, v.D, v.H, n, mi, reinterpret_cast<double*>(qpx_smem), G);
     } else {
       for (int e = tid; e < n * n; e += NT) {
         const int i = e / n, j = e - i * n;
@@ -1236,10 +1686,10 @@ __device__ __forceinline__ void stage_factor(const View& v, int dmode) {
 
 // Core of the reduced solve: given p1 (in v.u, length n) and v4, produce dy (v.dy) and dx.
 // dy = F^{-1}(A H^{-1} p1 - v4), dx = H^{-1}(p1 - A^T dy) with A H^{-1} = W^T L^{-1}.
-template <int NT>
+template <int NT, bool PF = false>
 __device__ void reduced_solve(const View& v, const double* v4, double* dx) {
   const int n = v.n, me = v.me, tid = threadIdx.x;
-  block_fsub<NT>(v.H, v.hrcp, n, v.u);  // u = L^{-1} p1
+  block_fsub<NT, PF>(v.H, v.hrcp, n, v.u);  // u = L^{-1} p1
   if (me > 0) {
     if (n >= kMfmaMinN && n <= kMfmaMaxN && me <= 64) {  // t = W^T u - v4 as column sums
       QPX_DYN_LDS(qpx_smem);
@@ -1264,7 +1714,7 @@ __device__ void reduced_solve(const View& v, const double* v4, double* dx) {
     }
     __syncthreads();
   }
-  block_bsub_t<NT>(v.H, v.hrcp, n, v.u);
+  block_bsub_t<NT, PF>(v.H, v.hrcp, n, v.u);
   for (int i = tid; i < n; i += NT) dx[i] = v.u[i];
   __syncthreads();
 }

@@ -68,7 +68,8 @@ __device__ void e_load(EView& e, const TS* Q, const TS* q, const TS* A, const TS
 // Initialisation (qpax/elastic_qp.py:9-47): H0 = Q + G^T G / 2,
 // x = H0^{-1}(-q - G^T (pen - h) / 2), z2 = (G x + pen - h) / 2, z1 = pen - z2, t = -z1, then the
 // CVXOPT shift on the stacked (z1, z2).
-template <int NT>
+// PF: the sweeps with L loaded ahead (block_fsub_tiled), for the one-workgroup-per-CU fused kernels
+template <int NT, bool PF = false>
 __device__ void e_init(EView& e, double* red) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
@@ -80,7 +81,7 @@ __device__ void e_init(EView& e, double* red) {
     v.u[i] = -v.q[i] - 0.5 * acc;
   }
   __syncthreads();
-  reduced_solve<NT>(v, v.b, v.x);
+  reduced_solve<NT, PF>(v, v.b, v.x);
   for (int c = tid; c < me; c += NT) v.y[c] = v.dy[c];
   double zmax = -INFINITY, zmin = INFINITY;
   for (int k = tid; k < mi; k += NT) {
@@ -180,7 +181,7 @@ __device__ void e_factor(EView& e, const GT* G) {
 
 // Algorithm 5 with right-hand side sg * (R1, R2, R3, R4, R5, R6, R7) (R7: the equality rows) using
 // the current factorisation.  Writes dx, dy (in v) and dt, ds1, ds2, dz1, dz2 (given arrays).
-template <int NT, typename GT>
+template <int NT, typename GT, bool PF = false>
 __device__ void e_solve(EView& e, const double* R1, const double* R2, const double* R3, const double* R4,
                         const double* R5, const double* R6, const double* R7, double sg, double* dt,
                         double* ds1, double* ds2, double* dz1, double* dz2, const GT* G) {
@@ -209,7 +210,7 @@ __device__ void e_solve(EView& e, const double* R1, const double* R2, const doub
   }
   for (int c = tid; c < me; c += NT) v.dy[c] = sg * R7[c];
   __syncthreads();
-  reduced_solve<NT>(v, v.dy, v.dx);
+  reduced_solve<NT, PF>(v, v.dy, v.dx);
   const bool wide = n >= kMfmaMinN;
   if (wide) {  // G dx into dz2 first
     rows_dot16<NT>(G, mi, n, v.dx, dz2);
@@ -293,9 +294,9 @@ template <int NT, typename GT>
 __device__ int e_pc_iteration(EView& e, double tol, double* red, const GT* G) {
   const int conv = e_residuals<NT>(e, 0.0, -1.0, 1.0, tol, red, G);
   e_factor<NT>(e, G);
-  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a, G);
+  e_solve<NT, GT, true>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a, G);
   e_center<NT>(e, red);
-  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
+  e_solve<NT, GT, true>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
   e_step<NT>(e, red);
   return conv;
 }
@@ -307,7 +308,7 @@ __device__ int e_relax_iteration(EView& e, double tol, double kappa, int consist
   const int conv = consistent ? e_residuals<NT>(e, kappa, -1.0, 1.0, tol, red, G)
                               : e_residuals<NT>(e, kappa, 1.0, -1.0, tol, red, G);
   e_factor<NT>(e, G);
-  e_solve<NT>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
+  e_solve<NT, GT, true>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
   e_step<NT>(e, red);
   return conv;
 }
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_e_fused_solve(Args<TS> a, int with_relax
   const int64_t p = blockIdx.x;
   EView e = eview_of(a, p);
   QPX_E_LOAD(NT, TS, e, a, p);
-  e_init<NT>(e, red);
+  e_init<NT, true>(e, red);
   // G straight from the caller's array (the workspace copy holds the same values widened to
   // float64): half the bytes of the per-iteration G passes for float32 inputs
   const TS* Gin = a.G + p * a.mi * a.n;
@@ -496,11 +497,11 @@ __device__ void e_bwd_load(const Args<TS>& a, EView& e, int64_t p) {
   __syncthreads();
 }
 
-template <int NT, typename TS>
+template <int NT, typename TS, bool PF = false>
 __device__ void e_bwd_solve_write(const Args<TS>& a, EView& e, int64_t p) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
-  e_solve<NT>(e, v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, v.G);
+  e_solve<NT, double, PF>(e, v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, v.G);
   const bool exact = a.flags & 1;
   for (int i = tid; i < n * n; i += NT) {
     const int r = i / n, c = i - r * n;
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(NT) void k_e_fused_bwd(Args<TS> a) {
   EView e = eview_of(a, p);
   e_bwd_load<NT, TS>(a, e, p);
   e_factor<NT>(e, e.v.G);
-  e_bwd_solve_write<NT, TS>(a, e, p);
+  e_bwd_solve_write<NT, TS, true>(a, e, p);
 }
 
 }  // namespace qpx

@@ -33,6 +33,8 @@
 #define QPX_PIN_S(x) asm volatile("" : "+r"(x))
 #define QPX_DYN_LDS(name) unsigned char* const name = ::emu::dyn_lds()
 #define QPX_GLOBAL_AS
+#define QPX_LDS_AS
+#define QPX_ASSUME_LDS(p) ((void)0)
 
 struct alignas(16) double2 {
   double x, y;
