@@ -326,8 +326,9 @@ __host__ __device__ inline int generic_lds_bytes(int n, int me, int nt) {
   if (n >= kMfmaMinN && n <= kMfmaMaxN && b < cs) b = cs;
   if (w_in_lds(n, me, nt) && 8 * n * me > b) b = 8 * n * me;
   if (n >= kMfmaMinN && n <= kMfmaMaxN) {
-    // the blocked Cholesky: two panels, W beside them when fused, then 1/L_jj and a column buffer
-    const int c = 8 * (2 * panel_words(n) + (w_fused(n, me, nt) ? n * me : 0) + n + 16);
+    // the blocked Cholesky: two panels, W beside them when fused, then 1/L_jj, a column buffer and
+    // the reference diagonal
+    const int c = 8 * (2 * panel_words(n) + (w_fused(n, me, nt) ? n * me : 0) + 2 * n + 16);
     if (c > b) b = c;
   }
   return b;
@@ -756,15 +757,22 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
 // rcpl (LDS).  Per column j, lanes 0..15 publish their unscaled entry of column j in colb (16
 // doubles of LDS) and read the whole column back, so every lane forms L_jj, 1/L_jj and the
 // multipliers a_cj / L_jj itself: one LDS write and read per column, no readlane chain.  The pivot
-// test's reference diagonal is read once per lane for all sixteen columns.
-__device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, const double* diag0_, double* rcp_,
-                                               double* rcpl_, double* colb_) {
+// test's reference diagonal (diag0, in LDS) is read once per lane for all sixteen columns.  1/L_jj
+// comes from v_rsq_f64 and two Newton steps (L_jj = d / L_jj): seven dependent operations on the
+// column's chain instead of a correctly rounded square root and division (about 22), within an
+// ulp or two of the rounded values.  A pivot must also be a normal number (d >= 2^-1022) for the
+// seed; a denormal one is replaced like one below the relative floor.  FULL: cw == 16 at compile
+// time, so the unrolled columns carry no width tests.
+template <bool FULL>
+__device__ __forceinline__ void panel_diag_cols(double* cur_, int c0, int cw, const double* diag0_, double* rcp_,
+                                                double* rcpl_, double* colb_) {
   QPX_LDS_AS double* cur = (QPX_LDS_AS double*)cur_;
   QPX_LDS_AS double* rcpl = (QPX_LDS_AS double*)rcpl_;
   QPX_LDS_AS double* colb = (QPX_LDS_AS double*)colb_;
-  const QPX_GLOBAL_AS double* diag0 = as_global(diag0_);
+  const QPX_LDS_AS double* diag0 = (const QPX_LDS_AS double*)diag0_;
   QPX_GLOBAL_AS double* rcp = as_global_w(rcp_);
   const int l = threadIdx.x & 63, r = l & 15;
+  if (FULL) cw = 16;
   double p[16], dv[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
@@ -773,7 +781,7 @@ __device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, con
   }
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    if (j < cw) {
+    if (FULL || j < cw) {
       if (l < 16) colb[r] = p[j];
       __builtin_amdgcn_wave_barrier();
       double a[16];
@@ -781,9 +789,14 @@ __device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, con
       for (int c = j; c < 16; ++c) a[c] = colb[c];
       __builtin_amdgcn_wave_barrier();
       const double d = a[j];
-      const bool ok = d > kPivotRel * fabs(dv[j]);
-      const double ljj = ok ? sqrt(d) : kHugePivot;
-      const double inv = 1.0 / ljj;
+      const bool ok = d > kPivotRel * fabs(dv[j]) && d >= 0x1p-1022;
+      // the seed and the Newton steps run on d unconditionally (whatever they give for a
+      // rejected pivot is discarded by the select)
+      const double h = 0.5 * d;
+      double y = __builtin_amdgcn_rsq(d);
+      y = fma(y, fma(-(h * y), y, 0.5), y);
+      y = fma(y, fma(-(h * y), y, 0.5), y);
+      const double inv = ok ? y : 1.0 / kHugePivot, ljj = ok ? d * y : kHugePivot;
       if (r > j) p[j] *= inv;
       if (r == j) p[j] = ljj;
       if (l == 0) {
@@ -793,13 +806,18 @@ __device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, con
 #pragma unroll
       for (int c = j + 1; c < 16; ++c) {
         const double lcj = a[c] * inv;
-        if (r >= c && c < cw) p[c] -= p[j] * lcj;
+        if (r >= c && (FULL || c < cw)) p[c] -= p[j] * lcj;
       }
     }
   }
   if (l < 16)
 #pragma unroll
     for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = p[c];
+}
+__device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, const double* diag0_, double* rcp_,
+                                               double* rcpl_, double* colb_) {
+  if (cw == 16) panel_diag_cols<true>(cur_, c0, cw, diag0_, rcp_, rcpl_, colb_);
+  else panel_diag_cols<false>(cur_, c0, cw, diag0_, rcp_, rcpl_, colb_);
 }
 
 // The rows below the diagonal block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i])
@@ -911,19 +929,20 @@ __device__ __forceinline__ void panel_w_trailing(double* Wl_, const double* cur_
   }
 }
 
-// Blocked Cholesky for 64 <= m <= kMfmaMaxN, same contract as block_cholesky.  Block column k
-// (16 columns, rows 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one
-// wavefront's registers, the rows below one per thread, and the panel is written back.  The
-// trailing tiles (I, J), k < J <= I, then receive -L_Ik L_Jk^T as four v_mfma_f64_16x16x4f64 each,
-// kTrailGroup tiles at a time per wavefront with the next group's loads in flight; the tiles of block
-// column k + 1 go to the second panel buffer, which is the next step's panel, the others back to
-// M.  With Wl (n x me in LDS beside the panels, holding A^T on entry), W = L^{-1} A^T is formed
-// alongside from each panel in the order block_w_lds uses, its rows of each block on wavefront 1
-// while the others form the panel's rows below the diagonal block.  P: two panels of
-// panel_words(m); aux: m + 16 doubles of LDS for 1/L_jj and the diagonal block's column broadcasts.
+// Blocked Cholesky for 64 <= m <= kMfmaMaxN, same contract as block_cholesky except that the
+// reference diagonal of the pivot test stays in LDS (no diag0).  Block column k (16 columns, rows
+// 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one wavefront's registers,
+// the rows below one per thread, and the panel is written back.  Trailing tiles (I, J) receive
+// -L_Ik L_Jk^T as four v_mfma_f64_16x16x4f64 per block column, two block columns per pass (the
+// step order is spelled out in the body), kTrailGroup tiles at a time per wavefront with the next
+// group's loads in flight.  With Wl (n x me in LDS beside the panels, holding A^T on entry),
+// W = L^{-1} A^T is formed alongside from each panel in the order block_w_lds uses, its rows of
+// each block on wavefront 1 while the others form the panel's rows below the diagonal block.
+// P: two panels of panel_words(m); aux: 2 m + 16 doubles of LDS for 1/L_jj, the diagonal block's
+// column broadcasts and the reference diagonal of the pivot test.
 constexpr int kTrailGroup = 4;  // trailing tiles per group: the next group's loads fly under its MFMAs
 template <int NT>
-__device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag0, double* rcp, double* P,
+__device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp, double* P,
                                                   double* Wl, int me, double* aux) {
   constexpr int W = NT / 64;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
@@ -937,7 +956,9 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
   QPX_GLOBAL_AS double* Mw = as_global_w(M);
   double* cur = P;
   double* nxt = P + panel_words(m);
-  for (int i = tid; i < m; i += NT) as_global_w(diag0)[i] = Mg[i * m + i];
+  // the reference diagonal of the pivot test, in LDS past 1/L_jj and the column buffer
+  double* dl = aux + m + 16;
+  for (int i = tid; i < m; i += NT) dl[i] = Mg[i * m + i];
   // rows r0.. of block column kb from M into a panel (zero above the diagonal and beyond the block
   // column's width), eight loads in flight per thread through clamped indices
   auto load_panel = [&](double* pnl_, int kb, int r0) {
@@ -959,7 +980,7 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
   };
   load_panel(cur, 0, 0);
   __syncthreads();
-  if (w == 0) panel_diag_lds(cur, 0, m < 16 ? m : 16, diag0, rcp, aux, aux + m);
+  if (w == 0) panel_diag_lds(cur, 0, m < 16 ? m : 16, dl, rcp, aux, aux + m);
   __syncthreads();
   auto load = [&](int I, int J, d4v& acc) {
 #pragma unroll
@@ -981,31 +1002,37 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
     }
   };
   // the trailing tiles (first + a, first + b), b <= a < tn2, idx >= i0 in row-major order, receive
-  // the updates of block column k (panel pk) and, with pk1, of block column k + 1 (panel pk1): this
-  // wavefront's tiles idx = i0 + wi, i0 + wi + ws, ..., kTrailGroup at a time, the next group
-  // loaded before the current group's MFMAs
-  auto trail = [&](const double* pk, const double* pk1, int k, int first, int tn2, int i0, int wi, int ws) {
-    const int T = tn2 * (tn2 + 1) / 2, R = m - 16 * k;
-    const int cnt = T > i0 + wi ? (T - i0 - wi + ws - 1) / ws : 0;
-    auto decode = [&](int t, int& I, int& J) {
-      const int idx = i0 + wi + ws * t < T ? i0 + wi + ws * t : T - 1;
+  // the updates of block column k (panel pk) and, with pk1, of block column k + 1 (panel pk1), in
+  // groups of kTrailGroup consecutive tiles that the wavefronts take from an LDS counter (so a
+  // wavefront that comes late takes fewer), the next group loaded before the current group's MFMAs
+  __shared__ int trail_next;
+  auto trail = [&](const double* pk, const double* pk1, int k, int first, int tn2, int i0) {
+    constexpr int GS = kTrailGroup;
+    const int T = tn2 * (tn2 + 1) / 2, R = m - 16 * k, ng = (T - i0 + GS - 1) / GS;
+    auto grab = [&]() {
+      int g = 0;
+      if (l == 0) g = atomicAdd(&trail_next, 1);
+      return __builtin_amdgcn_readfirstlane(g);
+    };
+    auto decode = [&](int g, int t, int& I, int& J) {
+      const int idx = i0 + GS * g + t < T ? i0 + GS * g + t : T - 1;
       int a = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
       while ((a + 1) * (a + 2) / 2 <= idx) ++a;
       while (a * (a + 1) / 2 > idx) --a;
       I = first + a;
       J = first + (idx - a * (a + 1) / 2);
     };
-    constexpr int GS = kTrailGroup;
+    int g = grab();
+    if (g >= ng) return;
     int tI[GS] = {}, tJ[GS] = {}, uI[GS] = {}, uJ[GS] = {};
     d4v acc[GS] = {}, nacc[GS] = {};
-    if (cnt > 0) {
 #pragma unroll
-      for (int t = 0; t < GS; ++t) { decode(t, tI[t], tJ[t]); load(tI[t], tJ[t], acc[t]); }
-    }
-    for (int pr = 0; pr < cnt; pr += GS) {
-      if (pr + GS < cnt) {
+    for (int t = 0; t < GS; ++t) { decode(g, t, tI[t], tJ[t]); load(tI[t], tJ[t], acc[t]); }
+    while (true) {
+      const int gn = grab();
+      if (gn < ng) {
 #pragma unroll
-        for (int t = 0; t < GS; ++t) { decode(pr + GS + t, uI[t], uJ[t]); load(uI[t], uJ[t], nacc[t]); }
+        for (int t = 0; t < GS; ++t) { decode(gn, t, uI[t], uJ[t]); load(uI[t], uJ[t], nacc[t]); }
       }
 #pragma unroll
       for (int t = 0; t < GS; ++t) {
@@ -1013,26 +1040,27 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
         if (pk1 != nullptr) update(pk1, k + 1, R - 16, tI[t], tJ[t], acc[t]);
       }
       // every group stores the same number of entries, so that the next group's loads can be
-      // awaited without waiting for these stores: entries outside the matrix or of slots past the
-      // wavefront's tiles go to M[0][m - 1], in the strict upper triangle, which nothing reads
+      // awaited without waiting for these stores: entries outside the matrix or past the last
+      // tile go to M[0][m - 1], in the strict upper triangle, which nothing reads
 #pragma unroll
       for (int t = 0; t < GS; ++t) {
-        const bool valid = pr + t < cnt;
+        const bool valid = i0 + GS * g + t < T;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
           Mw[valid && i < m && jj < m ? i * m + jj : m - 1] = acc[t][r];
         }
       }
+      if (gn >= ng) break;
 #pragma unroll
       for (int t = 0; t < GS; ++t) { acc[t] = nacc[t]; tI[t] = uI[t]; tJ[t] = uJ[t]; }
+      g = gn;
     }
   };
   // the first wavefront factors diagonal blocks, the others (all of them with one wavefront) the
   // rest of each phase
   const int w0 = W > 1 ? 1 : 0;
   const bool diag_wave = w == 0, work_wave = W == 1 || w > 0;
-  const int wi = W > 1 ? w - 1 : 0, ws = W > 1 ? W - 1 : 1;
   // Block columns in pairs (k, k + 1) with A = panel k (diagonal block factored) on entry:
   //   1. A's rows below the diagonal block (W's rows of block k beside them); store A
   //   2. block column k + 1 from M with block column k's update into B; its diagonal block factored
@@ -1040,8 +1068,9 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
   //   3. B's rows below the diagonal block (W's rows of block k + 1); then, while the first
   //      wavefront forms the diagonal block of block column k + 2 from both panels into A's first
   //      16 rows (free once A is stored; the trailing update reads A's rows from 32 and B's from
-  //      16) and factors it, the others store B and give every other trailing tile both panels'
-  //      updates in one pass through M, and apply block column k + 1 to W's rows below it
+  //      16) and factors it, the others store B and start giving every other trailing tile both
+  //      panels' updates in one pass through M (the first wavefront joins when its block is done);
+  //      then all apply block column k + 1 to W's rows below it
   //   4. the rows of block column k + 2 below its diagonal block from M into A
   // Each tile sees the updates of block columns 0, 1, ... in that order, as with one block column
   // per step, and the trailing matrix goes through M once per pair.
@@ -1071,10 +1100,11 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
       }
     }
     __syncthreads();
-    if (diag_wave) panel_diag_lds(B, c0 + 16, cw1, diag0, rcp, aux, aux + m);
+    if (diag_wave) panel_diag_lds(B, c0 + 16, cw1, dl, rcp, aux, aux + m);
     if (work_wave && has_w) panel_w_trailing<NT>(Wl, A, k, m, me, w0);
     __syncthreads();
     rows_of(B, k + 1);
+    if (tid == 0) trail_next = 0;
     __syncthreads();
     const bool more = k + 2 < nt;
     const int cw2 = more ? (R - 32 < 16 ? R - 32 : 16) : 0;
@@ -1089,13 +1119,11 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* diag
         if (c0 + 32 + pr_ < m) A[pr_ * kPanelLd + cq] = (cq < cw2 && cq <= pr_) ? acc[r] : 0.0;
       }
       __builtin_amdgcn_wave_barrier();
-      panel_diag_lds(A, c0 + 32, cw2, diag0, rcp, aux, aux + m);
+      panel_diag_lds(A, c0 + 32, cw2, dl, rcp, aux, aux + m);
     }
-    if (work_wave) {
-      panel_store<NT>(M, m, B, R - 16, c0 + 16, cw1, 64 * w0);
-      if (more) trail(A, B, k, k + 2, tn - 1, 1, wi, ws);
-      if (has_w) panel_w_trailing<NT>(Wl, B, k + 1, m, me, w0);
-    }
+    if (work_wave) panel_store<NT>(M, m, B, R - 16, c0 + 16, cw1, 64 * w0);
+    if (more) trail(A, B, k, k + 2, tn - 1, 1);
+    if (has_w) panel_w_trailing<NT>(Wl, B, k + 1, m, me, 0);
     __syncthreads();
     if (more) {
       load_panel(A, k + 2, 16);
@@ -1615,7 +1643,7 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
           Wl[e] = v.A[c * n + i];
         }
       __syncthreads();
-      block_cholesky_tiled<NT>(v.H, n, v.hdiag, v.hrcp, lds, Wl, me,
+      block_cholesky_tiled<NT>(v.H, n, v.hrcp, lds, Wl, me,
                                lds + 2 * panel_words(n) + (fused_w ? n * me : 0));
     } else {
       block_cholesky<NT>(v.H, n, v.hdiag, v.hrcp);

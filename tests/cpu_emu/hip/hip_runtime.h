@@ -207,6 +207,9 @@ inline void wave_barrier() { (void)wave_exchange(kWaveBarrier, nullptr, 0); }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) ::emu::mfma16((a), (b), (c))
 #define __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, x, y, z) ::emu::mfma4((a), (b), (c))
 #define __builtin_amdgcn_rcp(d) (1.0 / (d))
+// v_rsq_f64 is an approximation that callers refine: the seed here is off by 2^-20 so the
+// refinement is exercised
+#define __builtin_amdgcn_rsq(d) ((1.0 + 0x1p-20) / std::sqrt((double)(d)))
 #define __builtin_amdgcn_sched_barrier(m) ((void)0)
 #define __builtin_amdgcn_wave_barrier() ::emu::wave_barrier()
 #define __builtin_amdgcn_s_sleep(n) ((void)0)
