@@ -123,6 +123,39 @@ def test_elastic_c5_size_f32():
         assert e.max() < 1e-3
 
 
+@pytest.mark.parametrize("n,me,mi", [(200, 12, 260), (272, 16, 300)], ids=["n200-odd-blocks", "n272-max"])
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+def test_elastic_tiled_sizes_f64(n, me, mi, mode):
+    """Elastic QPs with equality rows at sizes whose blocked Cholesky has an odd number of 16-column
+    block columns: n = 200 (13, the last one 8 wide) and n = 272 (17, the largest tiled size), with
+    W = L^-1 A^T formed inside the factorisation, forward + backward in float64 against the oracle.
+    The relaxation runs with consistent signs: the reference's amplifies rounding differences
+    (about 1.98x per step over 30 steps), which would hide the factorisation's accuracy."""
+    import qpax_amd
+
+    Q, q, A, b, G, h, shifted = P.elastic_infeasible(2, n, mi, frac=0.5, seed=14, me=me)
+    pen = 14.0
+    print(f"[case] elastic tiled n={n} me={me} mi={mi} mode={mode}", flush=True)
+    ts = [torch.tensor(v, device="cuda", requires_grad=True) for v in (Q, q, A, b, G, h)]
+    x = qpax_amd.solve_qp_elastic_primal(ts[0], ts[1], ts[4], ts[5], pen, A=ts[2], b=ts[3], solver_tol=1e-7,
+                                         target_kappa=1e-3, mode=mode, relax="consistent")
+    ((x - 1.0) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    xo, rel, info = O.solve_qp_elastic_primal_fwd(Q, q, G, h, pen, solver_tol=1e-7, target_kappa=1e-3,
+                                                  consistent=True, A=A, b=b)
+    go = O.solve_qp_elastic_primal_bwd(Q, q, G, h, rel, 2 * (xo - 1), A=A)
+    ok = info[0] == 1
+    assert ok.all()
+    ex = rel_err(to_np(x)[ok], xo[ok])
+    print(f"x err max {ex.max():.2e}")
+    assert ex.max() < 1e-9  # measured 2.5e-11 (n = 200), 8e-13 (n = 272)
+    go = (go[0], go[1], go[4], go[5], go[2], go[3])  # oracle order (Q, q, G, h, A, b)
+    for nm, t, g in zip("Q q A b G h".split(), ts, go):
+        e = rel_err(to_np(t.grad)[ok], g[ok])
+        print(f"d{nm}: {e.max():.2e}")
+        assert e.max() < 1e-8  # measured at most 6.8e-11
+
+
 def test_elastic_equality_extension():
     import qpax_amd
 
