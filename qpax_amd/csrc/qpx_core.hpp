@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace qpx {
 
 constexpr int kMaxIters = 30;          // qpax/pdip.py:232, pdip_relaxed.py:76
@@ -48,6 +50,16 @@ __device__ __forceinline__ const QPX_GLOBAL_AS T* as_global(const T* p) {
 template <typename T>
 __device__ __forceinline__ QPX_GLOBAL_AS T* as_global_w(T* p) {
   return (QPX_GLOBAL_AS T*)p;
+}
+// A pointer that every lane holds alike (a workgroup's matrix), moved to scalar registers: loads
+// through it then take a scalar base and a 32-bit per-lane offset, one register per address
+// instead of a 64-bit sum.
+template <typename T>
+__device__ __forceinline__ const QPX_GLOBAL_AS T* uniform_global(const T* p) {
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (const QPX_GLOBAL_AS T*)(((uint64_t)hi << 32) | lo);
 }
 // A generic pointer parameter known to address LDS: with this assumption the compiler issues LDS
 // instructions through it instead of flat ones, which count against both the LDS and the
@@ -1248,35 +1260,45 @@ __device__ __noinline__ void fsub_lds(const double* L, const double* rcp, int m,
   QPX_LDS_AS double* b = (QPX_LDS_AS double*)b_;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, t = tid - 64;
   const int nt = (m + 15) / 16;
-  const QPX_GLOBAL_AS double* Lg = as_global(L);
-  const QPX_GLOBAL_AS double* rg = as_global(rcp);
+  const QPX_GLOBAL_AS double* Lg = uniform_global(L);
+  const QPX_GLOBAL_AS double* rg = uniform_global(rcp);
+  // a block with a predecessor and all 16 columns inside the matrix reads each row as one run
+  // (immediate offsets from one address); the first and a narrow last block clamp every index
   auto fetch = [&](int k, double (&v)[32], double& rl) {
     if (k >= nt) return;
     const int c0 = 16 * k;
+    const bool run = k > 0 && c0 + 16 <= m;
     if (w == 0) {
       const int cw = m - c0 < 16 ? m - c0 : 16, r = c0 + (l < cw ? l : cw - 1);
+      if (run) {
+        const QPX_GLOBAL_AS double* rp = Lg + (unsigned)(r * m + c0 - 16);
 #pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        const int c = c0 - 16 + j, cc = c < 0 ? 0 : (c < m ? c : m - 1);
-        v[j] = Lg[r * m + cc];
+        for (int j = 0; j < 32; ++j) v[j] = rp[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+          const int c = c0 - 16 + j, cc = c < 0 ? 0 : (c < m ? c : m - 1);
+          v[j] = Lg[(unsigned)(r * m + cc)];
+        }
       }
-      rl = rg[r];
+      rl = rg[(unsigned)r];
     } else {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int r = 16 * (k + 1) + t + h * NW, rr = r < m ? r : m - 1;
+        const QPX_GLOBAL_AS double* rp = Lg + (unsigned)(rr * m + (k > 0 ? 16 * (k - 1) : 0));
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const int c = 16 * (k - 1) + j;
-          v[16 * h + j] = Lg[rr * m + (c < 0 ? 0 : c)];
-        }
+        for (int j = 0; j < 16; ++j) v[16 * h + j] = rp[j];
       }
     }
   };
-  auto step = [&](int k, double (&v)[32], double& rl) {
-    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
+  // FULL: block k is 16 rows wide (every block but the last), so the elimination runs with no
+  // width tests (they would make each column a branch that waits for every load in flight)
+  auto step = [&](int k, double (&v)[32], double& rl, auto full) {
+    constexpr bool FULL = decltype(full)::value;
+    const int c0 = 16 * k, cw = FULL ? 16 : (m - c0 < 16 ? m - c0 : 16);
     if (w == 0) {
-      double bi = l < cw ? b[c0 + l] : 0.0;
+      double bi = (FULL || l < cw) ? b[c0 + l] : 0.0;
       if (k > 0) {
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
@@ -1288,9 +1310,9 @@ __device__ __noinline__ void fsub_lds(const double* L, const double* rcp, int m,
       }
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        if (j < cw) {
+        if (FULL || j < cw) {
           const double yj = readlane_d(bi * rl, j);
-          const double lv = (l > j && l < cw) ? v[16 + j] : 0.0;
+          const double lv = (l > j && (FULL || l < cw)) ? v[16 + j] : 0.0;
           bi = (l == j) ? yj : bi - lv * yj;
         }
       }
@@ -1316,9 +1338,20 @@ __device__ __noinline__ void fsub_lds(const double* L, const double* rcp, int m,
   double va[32], vb[32], ra = 0.0, rb = 0.0;
   fetch(0, va, ra);
   fetch(1, vb, rb);
-  for (int k = 0; k < nt; k += 2) {
-    step(k, va, ra);
-    if (k + 1 < nt) step(k + 1, vb, rb);
+  const std::integral_constant<bool, true> full;
+  const std::integral_constant<bool, false> part;
+  int k = 0;
+  for (; k + 2 < nt; k += 2) {  // blocks k and k + 1 both full
+    step(k, va, ra, full);
+    step(k + 1, vb, rb, full);
+  }
+  if (k < nt) {
+    if (k + 1 < nt) {
+      step(k, va, ra, full);
+      step(k + 1, vb, rb, part);
+    } else {
+      step(k, va, ra, part);
+    }
   }
 }
 
@@ -1332,8 +1365,10 @@ __device__ __noinline__ void bsub_t_lds(const double* L, const double* rcp, int 
   QPX_LDS_AS double* b = (QPX_LDS_AS double*)b_;
   const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, t = tid - 64;
   const int nt = (m + 15) / 16;
-  const QPX_GLOBAL_AS double* Lg = as_global(L);
-  const QPX_GLOBAL_AS double* rg = as_global(rcp);
+  const QPX_GLOBAL_AS double* Lg = uniform_global(L);
+  const QPX_GLOBAL_AS double* rg = uniform_global(rcp);
+  // the row index of every load is the same on all lanes (a scalar base), the column the per-lane
+  // offset
   auto fetch = [&](int k, double (&v)[32], double& rl) {
     if (k < 0) return;
     const int c0 = 16 * k;
@@ -1342,9 +1377,9 @@ __device__ __noinline__ void bsub_t_lds(const double* L, const double* rcp, int 
 #pragma unroll
       for (int j = 0; j < 32; ++j) {
         const int r = c0 + j;
-        v[j] = Lg[(r < m ? r : m - 1) * m + c];
+        v[j] = (Lg + (unsigned)((r < m ? r : m - 1) * m))[(unsigned)c];
       }
-      rl = rg[c];
+      rl = rg[(unsigned)c];
     } else {
       const int c1 = c0 + 16;
 #pragma unroll
@@ -1353,28 +1388,30 @@ __device__ __noinline__ void bsub_t_lds(const double* L, const double* rcp, int 
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int i = c1 + j;
-          v[16 * h + j] = Lg[(i < m ? i : m - 1) * m + rr];
+          v[16 * h + j] = (Lg + (unsigned)((i < m ? i : m - 1) * m))[(unsigned)rr];
         }
       }
     }
   };
-  auto step = [&](int k, double (&v)[32], double& rl) {
-    const int c0 = 16 * k, cw = m - c0 < 16 ? m - c0 : 16;
-    const int c1 = c0 + 16, cw1 = m - c1 < 16 ? m - c1 : 16;  // block k + 1 (when k + 1 < nt)
+  // FULL: blocks k and k + 1 are both 16 rows wide (k < nt - 2): no width tests
+  auto step = [&](int k, double (&v)[32], double& rl, auto full) {
+    constexpr bool FULL = decltype(full)::value;
+    const int c0 = 16 * k, cw = FULL ? 16 : (m - c0 < 16 ? m - c0 : 16);
+    const int c1 = c0 + 16, cw1 = FULL ? 16 : (m - c1 < 16 ? m - c1 : 16);  // block k + 1 (when k + 1 < nt)
     if (w == 0) {
-      double bi = l < cw ? b[c0 + l] : 0.0;
+      double bi = (FULL || l < cw) ? b[c0 + l] : 0.0;
       if (k + 1 < nt) {
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int j = 0; j < 16; j += 2) {
-          if (j < cw1) a0 += v[16 + j] * b[c1 + j];
-          if (j + 1 < cw1) a1 += v[16 + j + 1] * b[c1 + j + 1];
+          if (FULL || j < cw1) a0 += v[16 + j] * b[c1 + j];
+          if (FULL || j + 1 < cw1) a1 += v[16 + j + 1] * b[c1 + j + 1];
         }
         bi -= a0 + a1;
       }
 #pragma unroll
       for (int j = 15; j >= 0; --j) {
-        if (j < cw) {
+        if (FULL || j < cw) {
           const double xj = readlane_d(bi * rl, j);
           const double lv = (l < j) ? v[j] : 0.0;
           bi = (l == j) ? xj : bi - lv * xj;
@@ -1389,8 +1426,8 @@ __device__ __noinline__ void bsub_t_lds(const double* L, const double* rcp, int 
           double a0 = 0.0, a1 = 0.0;
 #pragma unroll
           for (int j = 0; j < 16; j += 2) {
-            if (j < cw1) a0 += v[16 * h + j] * b[c1 + j];
-            if (j + 1 < cw1) a1 += v[16 * h + j + 1] * b[c1 + j + 1];
+            if (FULL || j < cw1) a0 += v[16 * h + j] * b[c1 + j];
+            if (FULL || j + 1 < cw1) a1 += v[16 * h + j + 1] * b[c1 + j + 1];
           }
           b[r] -= a0 + a1;
         }
@@ -1402,10 +1439,18 @@ __device__ __noinline__ void bsub_t_lds(const double* L, const double* rcp, int 
   double va[32], vb[32], ra = 0.0, rb = 0.0;
   fetch(nt - 1, va, ra);
   fetch(nt - 2, vb, rb);
-  for (int k = nt - 1; k >= 0; k -= 2) {
-    step(k, va, ra);
-    if (k - 1 >= 0) step(k - 1, vb, rb);
+  const std::integral_constant<bool, true> full;
+  const std::integral_constant<bool, false> part;
+  // blocks nt - 1 and nt - 2 in the general form, then the full ones two at a time (their register
+  // sets keep alternating: block k uses va when nt - 1 - k is even)
+  step(nt - 1, va, ra, part);
+  if (nt - 2 >= 0) step(nt - 2, vb, rb, part);
+  int k = nt - 3;
+  for (; k - 1 >= 0; k -= 2) {
+    step(k, va, ra, full);
+    step(k - 1, vb, rb, full);
   }
+  if (k >= 0) step(k, va, ra, full);
 }
 
 // the one-barrier sweeps need every row below (above) a block on at most two threads of the
