@@ -61,6 +61,13 @@ __device__ __forceinline__ const QPX_GLOBAL_AS T* uniform_global(const T* p) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
   return (const QPX_GLOBAL_AS T*)(((uint64_t)hi << 32) | lo);
 }
+template <typename T>
+__device__ __forceinline__ QPX_GLOBAL_AS T* uniform_global_w(T* p) {
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (QPX_GLOBAL_AS T*)(((uint64_t)hi << 32) | lo);
+}
 // A generic pointer parameter known to address LDS: with this assumption the compiler issues LDS
 // instructions through it instead of flat ones, which count against both the LDS and the
 // vector-memory wait counters.
@@ -641,19 +648,21 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
   }
   GT reg[EPT];
   double dreg = 0.0;
-  const QPX_GLOBAL_AS GT* Gg = as_global(G);
+  const QPX_GLOBAL_AS GT* Gg = uniform_global(G);
+  const QPX_GLOBAL_AS double* Dg = uniform_global(D);
   const int64_t last = (int64_t)mi * n - 1;
   auto fetch = [&](int k0) {
     const int64_t base = (int64_t)k0 * n;
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
       const int64_t g = base + tid + NT * u;
-      reg[u] = Gg[g < last ? g : last];
+      reg[u] = Gg[(unsigned)(g < last ? g : last)];  // mi * n < 2^31
     }
     const int kd = k0 + (tid & (kGtdgRows - 1));
-    dreg = as_global(D)[kd < mi ? kd : mi - 1];
+    dreg = Dg[(unsigned)(kd < mi ? kd : mi - 1)];
   };
-  const QPX_GLOBAL_AS double* Qg = as_global(Q);
+  const QPX_GLOBAL_AS double* Qg = uniform_global(Q);
+  QPX_GLOBAL_AS double* Hw = uniform_global_w(H);
   for (int u0 = 0; u0 < U; u0 += W) {
     const int u = u0 + w;
     const bool full = u < nF;  // a full super-block, else a pair of diagonal ones (or nothing)
@@ -690,7 +699,7 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * tI[s] + 4 * r + rq, j = 16 * tJ[s] + cq;
-        acc[s][r] = Qg[(i < n ? i : n - 1) * n + (j < n ? j : n - 1)];
+        acc[s][r] = Qg[(unsigned)((i < n ? i : n - 1) * n + (j < n ? j : n - 1))];
       }
     __syncthreads();
     for (int e = tid; e < kGtdgRows * (ldl - n); e += NT) {
@@ -753,7 +762,7 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * tI[s] + 4 * r + rq, j = 16 * tJ[s] + cq;
-        if (i < n && j <= i) as_global_w(H)[i * n + j] = acc[s][r];
+        if (i < n && j <= i) Hw[(unsigned)(i * n + j)] = acc[s][r];
       }
     }
   }
@@ -867,11 +876,11 @@ __device__ __forceinline__ void panel_rows(double* cur_, int R, int c0, int cw, 
 // The factored panel into rows c0..m-1 of M's block column.
 template <int NT>
 __device__ __forceinline__ void panel_store(double* M_, int m, const double* cur_, int R, int c0, int cw, int t0 = 0) {
-  QPX_GLOBAL_AS double* M = as_global_w(M_);
+  QPX_GLOBAL_AS double* M = uniform_global_w(M_);
   const QPX_LDS_AS double* cur = (const QPX_LDS_AS double*)cur_;
   for (int e = (int)threadIdx.x - t0; e < R * 16; e += NT - t0) {
     const int r = e >> 4, c = e & 15;
-    if (c < cw && (r >= 16 || c <= r)) M[(c0 + r) * m + c0 + c] = cur[r * kPanelLd + c];
+    if (c < cw && (r >= 16 || c <= r)) M[(unsigned)((c0 + r) * m + c0 + c)] = cur[r * kPanelLd + c];
   }
 }
 
@@ -964,13 +973,13 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
   QPX_ASSUME_LDS(P);
   QPX_ASSUME_LDS(aux);
   QPX_ASSUME_LDS(Wl);
-  const QPX_GLOBAL_AS double* Mg = as_global(M);
-  QPX_GLOBAL_AS double* Mw = as_global_w(M);
+  const QPX_GLOBAL_AS double* Mg = uniform_global(M);
+  QPX_GLOBAL_AS double* Mw = uniform_global_w(M);
   double* cur = P;
   double* nxt = P + panel_words(m);
   // the reference diagonal of the pivot test, in LDS past 1/L_jj and the column buffer
   double* dl = aux + m + 16;
-  for (int i = tid; i < m; i += NT) dl[i] = Mg[i * m + i];
+  for (int i = tid; i < m; i += NT) dl[i] = Mg[(unsigned)(i * m + i)];
   // rows r0.. of block column kb from M into a panel (zero above the diagonal and beyond the block
   // column's width), eight loads in flight per thread through clamped indices
   auto load_panel = [&](double* pnl_, int kb, int r0) {
@@ -981,7 +990,7 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + tid + u * NT, r = e >> 4 < R ? e >> 4 : R - 1, c = (e & 15) < cw ? e & 15 : cw - 1;
-        val[u] = Mg[(c0 + r) * m + c0 + c];
+        val[u] = Mg[(unsigned)((c0 + r) * m + c0 + c)];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -998,7 +1007,7 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = 16 * I + 4 * r + rq, jj = 16 * J + cq;
-      acc[r] = Mg[(i < m ? i : m - 1) * m + (jj < m ? jj : m - 1)];
+      acc[r] = Mg[(unsigned)((i < m ? i : m - 1) * m + (jj < m ? jj : m - 1))];
     }
   };
   // acc -= L_Ik L_Jk^T from the panel of block column k (rows past m count as zero)
@@ -1060,7 +1069,7 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = 16 * tI[t] + 4 * r + rq, jj = 16 * tJ[t] + cq;
-          Mw[valid && i < m && jj < m ? i * m + jj : m - 1] = acc[t][r];
+          Mw[(unsigned)(valid && i < m && jj < m ? i * m + jj : m - 1)] = acc[t][r];
         }
       }
       if (gn >= ng) break;
