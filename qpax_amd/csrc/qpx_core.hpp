@@ -426,6 +426,71 @@ __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const d
   }
 }
 
+// out[k] = sum_j M[k n + j] x[j] for every row k < rows, as rows_dot16 (n <= kMfmaMaxN), and from
+// the same loads the column sums gcol[j] = sum_k M[k n + j] v[k]: each lane accumulates its
+// columns (cq, cq + 16, ...) over its rows, and the 4 W partial sums of a column meet through
+// `part` (LDS, 4 W x 16 ceil(n / 16) doubles), summed in a fixed order.  gcol may be in LDS or
+// global memory.  One read of M for both products.  Contains barriers.
+template <int NT, typename MT>
+__device__ __forceinline__ void rows_cols16(const MT* M, int rows, int n, const double* x, double* out,
+                                            const double* v, double* gcol, double* part) {
+  constexpr int W = NT / 64;
+  constexpr int S = (kMfmaMaxN + 15) / 16;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, rq = l >> 4, cq = l & 15;
+  const QPX_GLOBAL_AS MT* Mg = uniform_global(M);
+  const QPX_GLOBAL_AS double* vg = as_global(v);
+  const int P = 16 * ((n + 15) / 16);
+  double xs[S], ca[S];
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    const int j = cq + 16 * u;
+    xs[u] = j < n ? x[j] : 0.0;
+    ca[u] = 0.0;
+  }
+  for (int kb = 4 * w; kb < rows; kb += 8 * W) {
+    MT mv[2][S];
+    double vk[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int k0 = kb + 4 * W * g + rq, k = k0 < rows ? k0 : rows - 1;
+      const double vv = vg[(unsigned)k];
+      vk[g] = k0 < rows ? vv : 0.0;
+#pragma unroll
+      for (int u = 0; u < S; ++u) {
+        const int j = cq + 16 * u;
+        mv[g][u] = Mg[(unsigned)(k * n + (j < n ? j : n - 1))];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+      for (int u = 0; u < S; u += 2) {
+        if (cq + 16 * u < n) a0 += (double)mv[g][u] * xs[u];
+        if (u + 1 < S && cq + 16 * (u + 1) < n) a1 += (double)mv[g][u + 1] * xs[u + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < S; ++u) ca[u] += (double)mv[g][u] * vk[g];
+      const double sv = sum16(a0 + a1);
+      const int k = kb + 4 * W * g + rq;
+      if (cq == 0 && k < rows) out[k] = sv;
+    }
+  }
+  QPX_ASSUME_LDS(part);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < S; ++u)
+    if (16 * u < P) part[(4 * w + rq) * P + cq + 16 * u] = ca[u];
+  __syncthreads();
+  for (int c = tid; c < n; c += NT) {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4 * W; ++q) sum += part[q * P + c];
+    gcol[c] = sum;
+  }
+  __syncthreads();
+}
+
 // Column slots per lane of the column sums in the NT-thread generic kernels (n <= 96 runs on 128
 // threads, n <= kMfmaMaxN on 256), and the LDS they take for the partial sums: within the
 // dynamic LDS of those kernels (the G^T D G chunk, free outside the factorisation) for n >= 64.

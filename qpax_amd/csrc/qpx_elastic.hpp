@@ -117,21 +117,30 @@ __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   const bool wide = n >= kMfmaMinN;
+  // for the column-sum form below: G^T z2 from the same pass over G as G x, kept in LDS past the
+  // partial sums of both passes
+  constexpr int CPL = colsum_slots<NT>();
+  QPX_DYN_LDS(qpx_smem);
+  double* part = reinterpret_cast<double*>(qpx_smem);
+  const int gofs = 4 * (NT / 64) * 16 * ((n + 15) / 16), cofs = (NT / 64) * 64 * CPL;
+  double* gtz = part + (gofs > cofs ? gofs : cofs);
   if (wide) {  // G x into r6 and A x into r4 first; Q x by columns (Q is symmetric), coalesced
-    rows_dot16<NT>(G, mi, n, v.x, e.r6);
+    if (n <= kMfmaMaxN) rows_cols16<NT>(G, mi, n, v.x, e.r6, e.z2, gtz, part);
+    else rows_dot16<NT>(G, mi, n, v.x, e.r6);
     rows_dot16<NT>(v.A, me, n, v.x, v.r4);
     __syncthreads();
   }
   double m = 0.0;
-  if (wide && n <= kMfmaMaxN) {  // r1 = q + Q x + G^T z2 + A^T y as column sums (Q is symmetric)
-    constexpr int CPL = colsum_slots<NT>();
-    QPX_DYN_LDS(qpx_smem);
+  if (wide && n <= kMfmaMaxN) {  // r1 = q + Q x + A^T y as column sums (Q is symmetric), + G^T z2
     double acc[CPL] = {};
     colsum_acc<NT, CPL>(v.Q, n, n, v.x, acc);
-    colsum_acc<NT, CPL>(G, mi, n, e.z2, acc);
     colsum_acc<NT, CPL>(v.A, me, n, v.y, acc);
-    colsum_finish<NT, CPL>(acc, n, v.q, 1.0, 1.0, v.r1, reinterpret_cast<double*>(qpx_smem));
-    for (int i = tid; i < n; i += NT) m = nmax(m, fabs(v.r1[i]));
+    colsum_finish<NT, CPL>(acc, n, v.q, 1.0, 1.0, v.r1, part);
+    for (int i = tid; i < n; i += NT) {
+      const double r1 = v.r1[i] + gtz[i];
+      v.r1[i] = r1;
+      m = nmax(m, fabs(r1));
+    }
   } else {
     for (int i = tid; i < n; i += NT) {
       double acc = v.q[i];
