@@ -345,9 +345,9 @@ __host__ __device__ inline int generic_lds_bytes(int n, int me, int nt) {
   if (n >= kMfmaMinN && n <= kMfmaMaxN && b < cs) b = cs;
   if (w_in_lds(n, me, nt) && 8 * n * me > b) b = 8 * n * me;
   if (n >= kMfmaMinN && n <= kMfmaMaxN) {
-    // the blocked Cholesky: two panels, W beside them when fused, then 1/L_jj, a column buffer and
-    // the reference diagonal
-    const int c = 8 * (2 * panel_words(n) + (w_fused(n, me, nt) ? n * me : 0) + 2 * n + 16);
+    // the blocked Cholesky: two panels, W beside them when fused, then 1/L_jj, a column buffer, the
+    // reference diagonal and L_kk^{-1}
+    const int c = 8 * (2 * panel_words(n) + (w_fused(n, me, nt) ? n * me : 0) + 2 * n + 16 + 16 * kPanelLd);
     if (c > b) b = c;
   }
   return b;
@@ -848,18 +848,24 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
 // column's chain instead of a correctly rounded square root and division (about 22), within an
 // ulp or two of the rounded values.  A pivot must also be a normal number (d >= 2^-1022) for the
 // seed; a denormal one is replaced like one below the relative floor.  FULL: cw == 16 at compile
-// time, so the unrolled columns carry no width tests.
+// time, so the unrolled columns carry no width tests.  The block's inverse goes to linv (16 rows
+// at stride kPanelLd, zero past cw): lane r applies each column's scaling and eliminations, with
+// the multipliers it already forms, to column r of the identity, off the pivot chain.
 template <bool FULL>
 __device__ __forceinline__ void panel_diag_cols(double* cur_, int c0, int cw, const double* diag0_, double* rcp_,
-                                                double* rcpl_, double* colb_) {
+                                                double* rcpl_, double* colb_, double* linv_) {
   QPX_LDS_AS double* cur = (QPX_LDS_AS double*)cur_;
   QPX_LDS_AS double* rcpl = (QPX_LDS_AS double*)rcpl_;
   QPX_LDS_AS double* colb = (QPX_LDS_AS double*)colb_;
   const QPX_LDS_AS double* diag0 = (const QPX_LDS_AS double*)diag0_;
   QPX_GLOBAL_AS double* rcp = as_global_w(rcp_);
+  QPX_LDS_AS double* linv = (QPX_LDS_AS double*)linv_;
   const int l = threadIdx.x & 63, r = l & 15;
   if (FULL) cw = 16;
-  double p[16], dv[16];
+  // lane r also holds column r of L^{-1}, built by the same eliminations (identity at the start)
+  double p[16], dv[16], e[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) e[c] = c == r ? 1.0 : 0.0;
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
     p[c] = cur[r * kPanelLd + c];
@@ -889,56 +895,78 @@ __device__ __forceinline__ void panel_diag_cols(double* cur_, int c0, int cw, co
         rcp[c0 + j] = inv;
         rcpl[c0 + j] = inv;
       }
+      e[j] *= inv;
 #pragma unroll
       for (int c = j + 1; c < 16; ++c) {
         const double lcj = a[c] * inv;
         if (r >= c && (FULL || c < cw)) p[c] -= p[j] * lcj;
+        e[c] -= lcj * e[j];
       }
     }
   }
-  if (l < 16)
+  if (l < 16) {
 #pragma unroll
     for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = p[c];
+    // L^{-1}[j][r] (zero past the block's width)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) linv[j * kPanelLd + r] = (FULL || (j < cw && r < cw)) ? e[j] : 0.0;
+  }
 }
 __device__ __forceinline__ void panel_diag_lds(double* cur_, int c0, int cw, const double* diag0_, double* rcp_,
-                                               double* rcpl_, double* colb_) {
-  if (cw == 16) panel_diag_cols<true>(cur_, c0, cw, diag0_, rcp_, rcpl_, colb_);
-  else panel_diag_cols<false>(cur_, c0, cw, diag0_, rcp_, rcpl_, colb_);
+                                               double* rcpl_, double* colb_, double* linv_) {
+  if (cw == 16) panel_diag_cols<true>(cur_, c0, cw, diag0_, rcp_, rcpl_, colb_, linv_);
+  else panel_diag_cols<false>(cur_, c0, cw, diag0_, rcp_, rcpl_, colb_, linv_);
 }
 
-// The rows below the diagonal block, one per thread: L[r, j] = (P[r, j] - sum_{i<j} L[r, i] L[j, i])
-// / L_jj, the subtractions in the order of the right-looking scalar factorisation.
-// rcp: the LDS copy of 1/L_jj.  With skip >= 0, wavefront `skip` takes no rows (it forms W's rows
-// meanwhile).
-template <int NT>
-__device__ __forceinline__ void panel_rows(double* cur_, int R, int c0, int cw, const double* rcp_, int skip = -1) {
+// The panel's rows below the diagonal block, L[r, :] = P[r, :] L_kk^{-T}, and with Wl W's rows of
+// the block, W[c0.., :] = L_kk^{-1} W[c0.., :], as 16x16 tiles of four v_mfma_f64_16x16x4f64
+// each, in place: items (W's column tiles first, then the row tiles) go to wavefronts wi, wi + ws,
+// ...  Each tile reads its operands into registers before it writes, and no two tiles share
+// entries.
+__device__ __forceinline__ void panel_rows_mfma(double* cur_, int R, int c0, int cw, const double* linv_,
+                                                double* Wl_, int me, int wi, int ws) {
   QPX_LDS_AS double* cur = (QPX_LDS_AS double*)cur_;
-  const QPX_LDS_AS double* rcp = (const QPX_LDS_AS double*)rcp_;
-  const int tid = threadIdx.x, wv = tid >> 6;
-  if (wv == skip) return;
-  const int t = skip >= 0 && wv > skip ? tid - 64 : tid, nth = skip >= 0 ? NT - 64 : NT;
-  double rc[16];
+  const QPX_LDS_AS double* linv = (const QPX_LDS_AS double*)linv_;
+  QPX_LDS_AS double* Wl = (QPX_LDS_AS double*)Wl_;
+  const int l = threadIdx.x & 63, rq = l >> 4, cq = l & 15;
+  const int nw = Wl_ != nullptr ? (me + 15) / 16 : 0, nr = (R + 15) / 16 - 1;
+  for (int it = wi; it < nw + nr; it += ws) {
+    double av[4], bv[4];
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    if (it < nw) {  // W's columns 16 it .. 16 it + 15: L_kk^{-1} (A) times W's block rows (B)
+      const int col = 16 * it + cq, colc = col < me ? col : me - 1;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) rc[j] = rcp[c0 + (j < cw ? j : cw - 1)];
-  for (int r = 16 + t; r < R; r += nth) {
-    double x[16];
+      for (int sl = 0; sl < 4; ++sl) {
+        const int j = 4 * sl + rq;
+        av[sl] = linv[cq * kPanelLd + j];
+        const double wv = Wl[(c0 + (j < cw ? j : cw - 1)) * me + colc];
+        bv[sl] = j < cw ? wv : 0.0;
+      }
 #pragma unroll
-    for (int c = 0; c < 16; ++c) x[c] = cur[r * kPanelLd + c];
+      for (int sl = 0; sl < 4; ++sl) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[sl], bv[sl], acc, 0, 0, 0);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      if (j < cw) {
-        double vj = x[j];
+      for (int r = 0; r < 4; ++r) {
+        const int i = 4 * r + rq;
+        if (i < cw && col < me) Wl[(c0 + i) * me + col] = acc[r];
+      }
+    } else {  // panel rows 16 t .. 16 t + 15 (A) times L_kk^{-T} (B)
+      const int t = it - nw + 1;
 #pragma unroll
-        for (int i = 0; i < j; ++i) vj -= x[i] * cur[j * kPanelLd + i];
-        x[j] = vj * rc[j];
+      for (int sl = 0; sl < 4; ++sl) {
+        av[sl] = cur[(16 * t + cq) * kPanelLd + 4 * sl + rq];
+        bv[sl] = linv[cq * kPanelLd + 4 * sl + rq];
+      }
+#pragma unroll
+      for (int sl = 0; sl < 4; ++sl) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[sl], bv[sl], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * t + 4 * r + rq;
+        if (row < R) cur[row * kPanelLd + cq] = cq < cw ? acc[r] : 0.0;
       }
     }
-#pragma unroll
-    for (int c = 0; c < 16; ++c) cur[r * kPanelLd + c] = x[c];
   }
 }
 
-// The factored panel into rows c0..m-1 of M's block column.
 template <int NT>
 __device__ __forceinline__ void panel_store(double* M_, int m, const double* cur_, int R, int c0, int cw, int t0 = 0) {
   QPX_GLOBAL_AS double* M = uniform_global_w(M_);
@@ -949,31 +977,6 @@ __device__ __forceinline__ void panel_store(double* M_, int m, const double* cur
   }
 }
 
-// W's rows of this block (W = L^{-1} A^T, n x me in LDS): forward elimination with the diagonal
-// block on wavefront 0, lane per column of W.
-__device__ __forceinline__ void panel_w_rows(double* Wl_, const double* cur_, const double* rcp_, int c0, int cw,
-                                             int me) {
-  QPX_LDS_AS double* __restrict__ Wl = (QPX_LDS_AS double*)Wl_;
-  const QPX_LDS_AS double* __restrict__ cur = (const QPX_LDS_AS double*)cur_;
-  const QPX_LDS_AS double* __restrict__ rcp = (const QPX_LDS_AS double*)rcp_;
-  const int l = threadIdx.x & 63;
-  double rc[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) rc[i] = rcp[c0 + (i < cw ? i : cw - 1)];
-  for (int cg = 0; cg < me; cg += 64) {
-    const int c = cg + l;
-    double wj[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double wi = (i < cw && c < me) ? Wl[(c0 + i) * me + c] : 0.0;
-#pragma unroll
-      for (int j = 0; j < i; ++j) wi -= (i < cw ? cur[i * kPanelLd + j] : 0.0) * wj[j];
-      wi *= i < cw ? rc[i] : 0.0;
-      wj[i] = wi;
-      if (i < cw && c < me) Wl[(c0 + i) * me + c] = wi;
-    }
-  }
-}
 
 // W's rows below block k receive -L[rows, block] W[block, :] as 16x16 tiles (block_w_lds's order),
 // over wavefronts w0..W-1.
@@ -1017,15 +1020,16 @@ __device__ __forceinline__ void panel_w_trailing(double* Wl_, const double* cur_
 
 // Blocked Cholesky for 64 <= m <= kMfmaMaxN, same contract as block_cholesky except that the
 // reference diagonal of the pivot test stays in LDS (no diag0).  Block column k (16 columns, rows
-// 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one wavefront's registers,
-// the rows below one per thread, and the panel is written back.  Trailing tiles (I, J) receive
-// -L_Ik L_Jk^T as four v_mfma_f64_16x16x4f64 per block column, two block columns per pass (the
-// step order is spelled out in the body), kTrailGroup tiles at a time per wavefront with the next
-// group's loads in flight.  With Wl (n x me in LDS beside the panels, holding A^T on entry),
-// W = L^{-1} A^T is formed alongside from each panel in the order block_w_lds uses, its rows of
-// each block on wavefront 1 while the others form the panel's rows below the diagonal block.
-// P: two panels of panel_words(m); aux: 2 m + 16 doubles of LDS for 1/L_jj, the diagonal block's
-// column broadcasts and the reference diagonal of the pivot test.
+// 16k..m-1) is a panel in LDS: its 16x16 diagonal block is factored in one wavefront's registers
+// and inverted (L_kk^{-1}), the rows below become P L_kk^{-T} as MFMA tiles, and the panel is
+// written back.  Trailing tiles (I, J) receive -L_Ik L_Jk^T as four v_mfma_f64_16x16x4f64 per
+// block column, two block columns per pass (the step order is spelled out in the body),
+// kTrailGroup tiles at a time per wavefront with the next group's loads in flight.  With Wl
+// (n x me in LDS beside the panels, holding A^T on entry), W = L^{-1} A^T is formed alongside:
+// W's rows of each block become L_kk^{-1} times themselves (MFMA tiles beside the panel's rows),
+// and the rows below take the block's update.  P: two panels of panel_words(m); aux:
+// 2 m + 16 + 16 kPanelLd doubles of LDS for 1/L_jj, the diagonal block's column broadcasts, the
+// reference diagonal of the pivot test and L_kk^{-1}.
 constexpr int kTrailGroup = 4;  // trailing tiles per group: the next group's loads fly under its MFMAs
 template <int NT>
 __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp, double* P,
@@ -1044,6 +1048,8 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
   double* nxt = P + panel_words(m);
   // the reference diagonal of the pivot test, in LDS past 1/L_jj and the column buffer
   double* dl = aux + m + 16;
+  // L_kk^{-1} of the latest factored diagonal block (16 rows at stride kPanelLd) past it
+  double* linv = dl + m;
   for (int i = tid; i < m; i += NT) dl[i] = Mg[(unsigned)(i * m + i)];
   // rows r0.. of block column kb from M into a panel (zero above the diagonal and beyond the block
   // column's width), eight loads in flight per thread through clamped indices
@@ -1066,7 +1072,9 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
   };
   load_panel(cur, 0, 0);
   __syncthreads();
-  if (w == 0) panel_diag_lds(cur, 0, m < 16 ? m : 16, dl, rcp, aux, aux + m);
+  if (w == 0) {
+    panel_diag_lds(cur, 0, m < 16 ? m : 16, dl, rcp, aux, aux + m, linv);
+  }
   __syncthreads();
   auto load = [&](int I, int J, d4v& acc) {
 #pragma unroll
@@ -1164,8 +1172,7 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
   double* B = nxt;
   auto rows_of = [&](double* pnl, int kb) {
     const int c0 = 16 * kb, R = m - c0, cw = R < 16 ? R : 16;
-    if (has_w && w == 1) panel_w_rows(Wl, pnl, aux, c0, cw, me);
-    else panel_rows<NT>(pnl, R, c0, cw, aux, has_w ? 1 : -1);
+    panel_rows_mfma(pnl, R, c0, cw, linv, has_w ? Wl : nullptr, me, w, W);
   };
   int k = 0;
   for (; k + 1 < nt; k += 2) {
@@ -1186,7 +1193,9 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
       }
     }
     __syncthreads();
-    if (diag_wave) panel_diag_lds(B, c0 + 16, cw1, dl, rcp, aux, aux + m);
+    if (diag_wave) {
+      panel_diag_lds(B, c0 + 16, cw1, dl, rcp, aux, aux + m, linv);
+    }
     if (work_wave && has_w) panel_w_trailing<NT>(Wl, A, k, m, me, w0);
     __syncthreads();
     rows_of(B, k + 1);
@@ -1205,7 +1214,7 @@ __device__ __noinline__ void block_cholesky_tiled(double* M, int m, double* rcp,
         if (c0 + 32 + pr_ < m) A[pr_ * kPanelLd + cq] = (cq < cw2 && cq <= pr_) ? acc[r] : 0.0;
       }
       __builtin_amdgcn_wave_barrier();
-      panel_diag_lds(A, c0 + 32, cw2, dl, rcp, aux, aux + m);
+      panel_diag_lds(A, c0 + 32, cw2, dl, rcp, aux, aux + m, linv);
     }
     if (work_wave) panel_store<NT>(M, m, B, R - 16, c0 + 16, cw1, 64 * w0);
     if (more) trail(A, B, k, k + 2, tn - 1, 1);
