@@ -843,7 +843,7 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
 // rcpl (LDS).  Per column j, lanes 0..15 publish their unscaled entry of column j in colb (16
 // doubles of LDS) and read the whole column back, so every lane forms L_jj, 1/L_jj and the
 // multipliers a_cj / L_jj itself: one LDS write and read per column, no readlane chain.  The pivot
-// test's reference diagonal (diag0, in LDS) is read once per lane for all sixteen columns.  1/L_jj
+// test's reference diagonal (diag0, in LDS) is read per column beside the column.  1/L_jj
 // comes from v_rsq_f64 and two Newton steps (L_jj = d / L_jj): seven dependent operations on the
 // column's chain instead of a correctly rounded square root and division (about 22), within an
 // ulp or two of the rounded values.  A pivot must also be a normal number (d >= 2^-1022) for the
@@ -863,14 +863,11 @@ __device__ __forceinline__ void panel_diag_cols(double* cur_, int c0, int cw, co
   const int l = threadIdx.x & 63, r = l & 15;
   if (FULL) cw = 16;
   // lane r also holds column r of L^{-1}, built by the same eliminations (identity at the start)
-  double p[16], dv[16], e[16];
+  double p[16], e[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) e[c] = c == r ? 1.0 : 0.0;
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    p[c] = cur[r * kPanelLd + c];
-    dv[c] = diag0[c0 + (c < cw ? c : cw - 1)];
-  }
+  for (int c = 0; c < 16; ++c) p[c] = cur[r * kPanelLd + c];
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     if (FULL || j < cw) {
@@ -881,7 +878,8 @@ __device__ __forceinline__ void panel_diag_cols(double* cur_, int c0, int cw, co
       for (int c = j; c < 16; ++c) a[c] = colb[c];
       __builtin_amdgcn_wave_barrier();
       const double d = a[j];
-      const bool ok = d > kPivotRel * fabs(dv[j]) && d >= 0x1p-1022;
+      // the reference diagonal from LDS (a broadcast read beside the column's)
+      const bool ok = d > kPivotRel * fabs(diag0[c0 + j]) && d >= 0x1p-1022;
       // the seed and the Newton steps run on d unconditionally (whatever they give for a
       // rejected pivot is discarded by the select)
       const double h = 0.5 * d;
