@@ -834,6 +834,64 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
   __syncthreads();
 }
 
+// block_cholesky for m <= kSmallChol (the Schur complement F = A H^{-1} A^T) on one wavefront:
+// lane r holds row r, and each column reaches every lane through a 32-double LDS broadcast
+// instead of global round trips and two workgroup barriers per column.  Same arithmetic as
+// block_cholesky (IEEE square root and division, the same products in the same order), so L is
+// the same bit for bit.  S: (m x 33 + 32) doubles of free LDS.
+constexpr int kSmallChol = 32;
+template <int NT>
+__device__ void block_cholesky_small(double* M_, int m, double* diag0_, double* rcp_, double* S_) {
+  QPX_ASSUME_LDS(S_);
+  QPX_LDS_AS double* S = (QPX_LDS_AS double*)S_;
+  QPX_LDS_AS double* colb = S + kSmallChol * 33;
+  QPX_GLOBAL_AS double* M = as_global_w(M_);
+  QPX_GLOBAL_AS double* diag0 = as_global_w(diag0_);
+  QPX_GLOBAL_AS double* rcp = as_global_w(rcp_);
+  const int tid = threadIdx.x, l = tid & 63;
+  __syncthreads();
+  for (int e = tid; e < m * m; e += NT) {
+    const int r = e / m, c = e - r * m;
+    S[r * 33 + c] = M[e];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int r = l < m ? l : m - 1;
+    double p[kSmallChol];
+#pragma unroll
+    for (int c = 0; c < kSmallChol; ++c) p[c] = c < m ? S[r * 33 + c] : 0.0;
+    if (l < m) diag0[l] = S[l * 33 + l];
+#pragma unroll
+    for (int j = 0; j < kSmallChol; ++j) {
+      if (j < m) {
+        if (l < m) colb[l] = p[j];
+        __builtin_amdgcn_wave_barrier();
+        double a[kSmallChol];
+#pragma unroll
+        for (int c = j; c < kSmallChol; ++c) a[c] = c < m ? colb[c] : 0.0;
+        __builtin_amdgcn_wave_barrier();
+        const double d = a[j];
+        const bool ok = d > kPivotRel * fabs(S[j * 33 + j]);
+        const double ljj = ok ? sqrt(d) : kHugePivot;
+        const double inv = 1.0 / ljj;
+        if (r > j) p[j] *= inv;
+        if (r == j) p[j] = ljj;
+        if (l == 0) rcp[j] = inv;
+#pragma unroll
+        for (int c = j + 1; c < kSmallChol; ++c) {
+          const double lcj = a[c] * inv;
+          if (c < m && r >= c) p[c] -= p[j] * lcj;
+        }
+      }
+    }
+    if (l < m)
+#pragma unroll
+      for (int c = 0; c < kSmallChol; ++c)
+        if (c < m) M[l * m + c] = p[c];
+  }
+  __syncthreads();
+}
+
 // Pieces of the blocked Cholesky (64 <= m <= kMfmaMaxN) on block column k, held as a panel in LDS:
 // rows c0..m-1 of columns c0..c0+cw-1 at row stride kPanelLd, zero above the diagonal and beyond cw
 // columns.
@@ -1783,7 +1841,9 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
     wtw_mfma<NT>(Wl, n, me, v.F);
     for (int e = tid; e < n * me; e += NT) v.W[e] = Wl[e];
     __syncthreads();
-    block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
+    // the Cholesky panels' LDS is free now
+    if (me <= kSmallChol) block_cholesky_small<NT>(v.F, me, v.fdiag, v.frcp, reinterpret_cast<double*>(qpx_smem));
+    else block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
     return;
   }
   if (w_in_lds(n, me, NT)) {
@@ -1801,7 +1861,9 @@ __device__ __forceinline__ void stage_factor_g(const View& v, int dmode, const G
     wtw_mfma<NT>(Wl, n, me, v.F);
     for (int e = tid; e < n * me; e += NT) v.W[e] = Wl[e];
     __syncthreads();
-    block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
+    // W's LDS copy is no longer read
+    if (me <= kSmallChol) block_cholesky_small<NT>(v.F, me, v.fdiag, v.frcp, Wl);
+    else block_cholesky<NT>(v.F, me, v.fdiag, v.frcp);
     return;
   }
   // W = L^{-1} A^T, all me columns at once: column-oriented forward substitution.
