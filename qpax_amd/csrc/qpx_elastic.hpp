@@ -111,33 +111,39 @@ __device__ void e_init(EView& e, double* red) {
 // Residuals (qpax/elastic_qp.py:105-110; relaxed: :256-261).  The reference's relaxed residuals
 // flip the sign of t and of r2 (sign_t = +1, sign_r2 = -1); the forward uses sign_t = -1,
 // sign_r2 = +1.  Returns the entry convergence flag.
-template <int NT, typename GT>
+template <int NT, typename GT, bool PF = false>
 __device__ int e_residuals(EView& e, double kappa, double sign_t, double sign_r2, double tol, double* red,
                            const GT* G) {
   const View& v = e.v;
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   const bool wide = n >= kMfmaMinN;
-  // for the column-sum form below: G^T z2 from the same pass over G as G x, kept in LDS past the
-  // partial sums of both passes
+  // PF (the fused kernels): G^T z2 from the same pass over G as G x (rows_cols16), kept in LDS past
+  // the partial sums of both passes; the per-stage kernels keep two passes and fewer registers
   constexpr int CPL = colsum_slots<NT>();
   QPX_DYN_LDS(qpx_smem);
   double* part = reinterpret_cast<double*>(qpx_smem);
   const int gofs = 4 * (NT / 64) * 16 * ((n + 15) / 16), cofs = (NT / 64) * 64 * CPL;
   double* gtz = part + (gofs > cofs ? gofs : cofs);
+  const bool fuse = PF && wide && n <= kMfmaMaxN;
   if (wide) {  // G x into r6 and A x into r4 first; Q x by columns (Q is symmetric), coalesced
-    if (n <= kMfmaMaxN) rows_cols16<NT>(G, mi, n, v.x, e.r6, e.z2, gtz, part);
-    else rows_dot16<NT>(G, mi, n, v.x, e.r6);
+    if constexpr (PF) {
+      if (n <= kMfmaMaxN) rows_cols16<NT>(G, mi, n, v.x, e.r6, e.z2, gtz, part);
+      else rows_dot16<NT>(G, mi, n, v.x, e.r6);
+    } else {
+      rows_dot16<NT>(G, mi, n, v.x, e.r6);
+    }
     rows_dot16<NT>(v.A, me, n, v.x, v.r4);
     __syncthreads();
   }
   double m = 0.0;
-  if (wide && n <= kMfmaMaxN) {  // r1 = q + Q x + A^T y as column sums (Q is symmetric), + G^T z2
+  if (wide && n <= kMfmaMaxN) {  // r1 = q + Q x + G^T z2 + A^T y as column sums (Q is symmetric)
     double acc[CPL] = {};
     colsum_acc<NT, CPL>(v.Q, n, n, v.x, acc);
+    if (!fuse) colsum_acc<NT, CPL>(G, mi, n, e.z2, acc);
     colsum_acc<NT, CPL>(v.A, me, n, v.y, acc);
     colsum_finish<NT, CPL>(acc, n, v.q, 1.0, 1.0, v.r1, part);
     for (int i = tid; i < n; i += NT) {
-      const double r1 = v.r1[i] + gtz[i];
+      const double r1 = fuse ? v.r1[i] + gtz[i] : v.r1[i];
       v.r1[i] = r1;
       m = nmax(m, fabs(r1));
     }
@@ -301,7 +307,7 @@ __device__ void e_step(EView& e, double* red) {
 // One predictor-corrector iteration (qpax/elastic_qp.py:99-174); returns the entry flag.
 template <int NT, typename GT>
 __device__ int e_pc_iteration(EView& e, double tol, double* red, const GT* G) {
-  const int conv = e_residuals<NT>(e, 0.0, -1.0, 1.0, tol, red, G);
+  const int conv = e_residuals<NT, GT, true>(e, 0.0, -1.0, 1.0, tol, red, G);
   e_factor<NT>(e, G);
   e_solve<NT, GT, true>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1a, e.ds2a, e.dz1a, e.dz2a, G);
   e_center<NT>(e, red);
@@ -314,8 +320,8 @@ __device__ int e_pc_iteration(EView& e, double tol, double* red, const GT* G) {
 // signs of t and r2.
 template <int NT, typename GT>
 __device__ int e_relax_iteration(EView& e, double tol, double kappa, int consistent, double* red, const GT* G) {
-  const int conv = consistent ? e_residuals<NT>(e, kappa, -1.0, 1.0, tol, red, G)
-                              : e_residuals<NT>(e, kappa, 1.0, -1.0, tol, red, G);
+  const int conv = consistent ? e_residuals<NT, GT, true>(e, kappa, -1.0, 1.0, tol, red, G)
+                              : e_residuals<NT, GT, true>(e, kappa, 1.0, -1.0, tol, red, G);
   e_factor<NT>(e, G);
   e_solve<NT, GT, true>(e, e.v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, e.v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, G);
   e_step<NT>(e, red);
