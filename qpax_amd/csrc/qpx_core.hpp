@@ -704,7 +704,9 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
   const int nt = (n + 15) / 16, ldl = gtdg_stride(n), ns = (nt + 3) / 4;
   const int nF = ns * (ns - 1) / 2, U = nF + (ns + 1) / 2;
   const int csz = kGtdgRows * n;
-  double* cD = chunk + kGtdgRows * ldl;
+  // the chunk keeps G's own element type (widened exactly when read as an operand), then D
+  GT* cG = reinterpret_cast<GT*>(chunk);
+  double* cD = chunk + (kGtdgRows * ldl * (int)sizeof(GT) + 7) / 8;
   int dst[EPT];
 #pragma unroll
   for (int u = 0; u < EPT; ++u) {
@@ -769,7 +771,7 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
     __syncthreads();
     for (int e = tid; e < kGtdgRows * (ldl - n); e += NT) {
       const int r = e / (ldl - n);
-      chunk[r * ldl + n + (e - r * (ldl - n))] = 0.0;
+      cG[r * ldl + n + (e - r * (ldl - n))] = (GT)0;
     }
     fetch(0);
     for (int k0 = 0; k0 < mi; k0 += kGtdgRows) {
@@ -778,7 +780,7 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
         const int64_t base = (int64_t)k0 * n;
 #pragma unroll
         for (int u = 0; u < EPT; ++u)
-          if (dst[u] >= 0) chunk[dst[u]] = base + tid + NT * u <= last ? (double)reg[u] : 0.0;
+          if (dst[u] >= 0) cG[dst[u]] = base + tid + NT * u <= last ? reg[u] : (GT)0;
         if (tid < kGtdgRows) cD[tid] = k0 + tid < mi ? dreg : 0.0;
       }
       __syncthreads();
@@ -787,15 +789,15 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
       if (full) {
 #pragma unroll
         for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
-          const double* gr = chunk + (4 * ks + rq) * ldl + cq;
-          const double* ga = gr + 64 * P;
-          const double* gb = gr + 64 * Qb;
+          const GT* gr = cG + (4 * ks + rq) * ldl + cq;
+          const GT* ga = gr + 64 * P;
+          const GT* gb = gr + 64 * Qb;
           const double dk = cD[4 * ks + rq];
           double a[4], b[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            a[i] = ga[16 * i] * dk;
-            b[i] = gb[16 * i];
+            a[i] = (double)ga[16 * i] * dk;
+            b[i] = (double)gb[16 * i];
           }
 #pragma unroll
           for (int s = 0; s < 16; ++s)
@@ -804,13 +806,13 @@ __device__ __noinline__ void gtdg_sb(const double* Q, const double* D, double* H
       } else {
 #pragma unroll
         for (int ks = 0; ks < kGtdgRows / 4; ++ks) {
-          const double* gr = chunk + (4 * ks + rq) * ldl + cq;
-          const double* g1 = gr + 64 * P;
+          const GT* gr = cG + (4 * ks + rq) * ldl + cq;
+          const GT* g1 = gr + 64 * P;
           const double dk = cD[4 * ks + rq];
           double b1[8], a1[8];
 #pragma unroll
           for (int i = 0; i < 8; ++i) {
-            b1[i] = g1[16 * i];  // column blocks 4P .. 4P + 7: both diagonal super-blocks
+            b1[i] = (double)g1[16 * i];  // column blocks 4P .. 4P + 7: both diagonal super-blocks
             a1[i] = b1[i] * dk;
           }
 #pragma unroll
