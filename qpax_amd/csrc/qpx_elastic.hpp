@@ -518,22 +518,60 @@ __device__ void e_bwd_solve_write(const Args<TS>& a, EView& e, int64_t p) {
   const int n = v.n, me = v.me, mi = v.mi, tid = threadIdx.x;
   e_solve<NT, double, PF>(e, v.r1, e.r2, e.r3, e.r4e, e.r5, e.r6, v.r4, -1.0, e.dt, e.ds1, e.ds2, e.dz1, e.dz2, v.G);
   const bool exact = a.flags & 1;
-  for (int i = tid; i < n * n; i += NT) {
-    const int r = i / n, c = i - r * n;
-    a.dQ[p * n * n + i] = (TS)(0.5 * (v.dx[r] * v.x[c] + v.x[r] * v.dx[c]));
+  // x, dx, y, dy and each row's lam, dlam (dlt with `exact`) in LDS, read as broadcasts; a thread
+  // writes one column of dQ, dG and dA (coalesced along the rows, no division per element).  Where
+  // the kernel's dynamic LDS is too small for them (small problems), one element per thread.
+  if (8 * (2 * n + 2 * me + 2 * mi) > generic_lds_bytes(n, me, NT)) {
+    for (int i = tid; i < n * n; i += NT) {
+      const int r = i / n, c = i - r * n;
+      a.dQ[p * n * n + i] = (TS)(0.5 * (v.dx[r] * v.x[c] + v.x[r] * v.dx[c]));
+    }
+    for (int i = tid; i < mi * n; i += NT) {
+      const int k = i / n, j = i - k * n;
+      const double lam = e.z2[k], dlt = e.dz2[k], dlam = dlt / lam;
+      const double val = exact ? dlt * v.x[j] + lam * v.dx[j] : lam * (dlam * v.x[j] + lam * v.dx[j]);
+      a.dG[(int64_t)p * mi * n + i] = (TS)val;
+    }
+    for (int i = tid; i < me * n; i += NT) {
+      const int c = i / n, j = i - c * n;
+      a.dA[(int64_t)p * me * n + i] = (TS)(v.dy[c] * v.x[j] + v.y[c] * v.dx[j]);
+    }
+  } else {
+    QPX_DYN_LDS(qpx_smem);
+    double* sx = reinterpret_cast<double*>(qpx_smem);
+    double* sdx = sx + n;
+    double* sy = sdx + n;
+    double* sdy = sy + me;
+    double* slam = sdy + me;
+    double* sdl = slam + mi;
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+      sx[i] = v.x[i];
+      sdx[i] = v.dx[i];
+    }
+    for (int c = tid; c < me; c += NT) {
+      sy[c] = v.y[c];
+      sdy[c] = v.dy[c];
+    }
+    for (int k = tid; k < mi; k += NT) {
+      const double lam = e.z2[k], dlt = e.dz2[k];
+      slam[k] = lam;
+      sdl[k] = exact ? dlt : dlt / lam;
+    }
+    __syncthreads();
+    for (int c = tid; c < n; c += NT) {
+      const double xc = sx[c], dxc = sdx[c];
+      for (int r = 0; r < n; ++r) a.dQ[(int64_t)p * n * n + r * n + c] = (TS)(0.5 * (sdx[r] * xc + sx[r] * dxc));
+      for (int k = 0; k < mi; ++k) {
+        const double lam = slam[k], d = sdl[k];
+        const double val = exact ? d * xc + lam * dxc : lam * (d * xc + lam * dxc);
+        a.dG[(int64_t)p * mi * n + k * n + c] = (TS)val;
+      }
+      for (int cc = 0; cc < me; ++cc) a.dA[(int64_t)p * me * n + cc * n + c] = (TS)(sdy[cc] * xc + sy[cc] * dxc);
+    }
   }
   for (int i = tid; i < n; i += NT) a.dq[p * n + i] = (TS)v.dx[i];
-  for (int i = tid; i < mi * n; i += NT) {
-    const int k = i / n, j = i - k * n;
-    const double lam = e.z2[k], dlt = e.dz2[k], dlam = dlt / lam;
-    const double val = exact ? dlt * v.x[j] + lam * v.dx[j] : lam * (dlam * v.x[j] + lam * v.dx[j]);
-    a.dG[(int64_t)p * mi * n + i] = (TS)val;
-  }
   for (int k = tid; k < mi; k += NT) a.dh[p * mi + k] = (TS)(-e.z2[k] * (e.dz2[k] / e.z2[k]));
-  for (int i = tid; i < me * n; i += NT) {
-    const int c = i / n, j = i - c * n;
-    a.dA[(int64_t)p * me * n + i] = (TS)(v.dy[c] * v.x[j] + v.y[c] * v.dx[j]);
-  }
   for (int c = tid; c < me; c += NT) a.db[p * me + c] = (TS)(-v.dy[c]);
 }
 
