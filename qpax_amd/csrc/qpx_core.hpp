@@ -385,10 +385,12 @@ __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const d
       const int j = cq + 16 * u;
       xs[u] = j < n ? x[j] : 0.0;
     }
-    for (int kb = 4 * w; kb < rows; kb += 8 * W) {
-      MT mv[2][S];
+    // NG groups of four rows per wavefront in flight: four for 4-byte elements, two for 8-byte
+    constexpr int NG = sizeof(MT) <= 4 ? 4 : 2;
+    for (int kb = 4 * w; kb < rows; kb += 4 * NG * W) {
+      MT mv[NG][S];
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
+      for (int g = 0; g < NG; ++g) {
         const int k0 = kb + 4 * W * g + rq, k = k0 < rows ? k0 : rows - 1;
 #pragma unroll
         for (int u = 0; u < S; ++u) {
@@ -397,7 +399,7 @@ __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const d
         }
       }
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
+      for (int g = 0; g < NG; ++g) {
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int u = 0; u < S; u += 2) {
@@ -502,18 +504,20 @@ static_assert(colsum_lds_bytes<128>() <= 8 * (kGtdgRows * 80 + kGtdgRows), "colu
 static_assert(colsum_lds_bytes<256>() <= 8 * (kGtdgRows * 112 + kGtdgRows), "column sums exceed the n = 97 chunk");
 
 // Column sums acc[u] += sum_k M[k, c] v[k] for the columns c = l + 64 u (< cols <= 64 CPL) of this
-// lane: the wavefronts take rows w, w + W, ..., eight rows' loads in flight per lane (clamped
-// indices; rows past the end add nothing).  Partial sums per wavefront: finish with colsum_finish.
+// lane: the wavefronts take rows w, w + W, ..., sixteen (4-byte elements) or eight rows' loads in
+// flight per lane (clamped indices; rows past the end add nothing).  Partial sums per wavefront: finish with colsum_finish.
 template <int NT, int CPL, typename T>
 __device__ __forceinline__ void colsum_acc(const T* M, int rows, int cols, const double* v, double (&acc)[CPL]) {
   constexpr int W = NT / 64;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const QPX_GLOBAL_AS T* Mg = as_global(M);
-  for (int k0 = w; k0 < rows; k0 += 8 * W) {
-    T m[8][CPL];
-    double vk[8];
+  // RD rows' loads in flight per lane: sixteen for 4-byte elements, eight for 8-byte
+  constexpr int RD = sizeof(T) <= 4 ? 16 : 8;
+  for (int k0 = w; k0 < rows; k0 += RD * W) {
+    T m[RD][CPL];
+    double vk[RD];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < RD; ++r) {
       const int k = k0 + r * W < rows ? k0 + r * W : rows - 1;
       vk[r] = v[k];
 #pragma unroll
@@ -523,7 +527,7 @@ __device__ __forceinline__ void colsum_acc(const T* M, int rows, int cols, const
       }
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
+    for (int r = 0; r < RD; ++r)
       if (k0 + r * W < rows)
 #pragma unroll
         for (int u = 0; u < CPL; ++u)
