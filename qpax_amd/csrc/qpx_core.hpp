@@ -385,8 +385,8 @@ __device__ __forceinline__ void rows_dot16(const MT* M, int rows, int n, const d
       const int j = cq + 16 * u;
       xs[u] = j < n ? x[j] : 0.0;
     }
-    // NG groups of four rows per wavefront in flight: four for 4-byte elements, two for 8-byte
-    constexpr int NG = sizeof(MT) <= 4 ? 4 : 2;
+    // NG groups of four rows per wavefront in flight: eight for 4-byte elements, two for 8-byte
+    constexpr int NG = sizeof(MT) <= 4 ? 8 : 2;
     for (int kb = 4 * w; kb < rows; kb += 4 * NG * W) {
       MT mv[NG][S];
 #pragma unroll
@@ -504,15 +504,16 @@ static_assert(colsum_lds_bytes<128>() <= 8 * (kGtdgRows * 80 + kGtdgRows), "colu
 static_assert(colsum_lds_bytes<256>() <= 8 * (kGtdgRows * 112 + kGtdgRows), "column sums exceed the n = 97 chunk");
 
 // Column sums acc[u] += sum_k M[k, c] v[k] for the columns c = l + 64 u (< cols <= 64 CPL) of this
-// lane: the wavefronts take rows w, w + W, ..., sixteen (4-byte elements) or eight rows' loads in
-// flight per lane (clamped indices; rows past the end add nothing).  Partial sums per wavefront: finish with colsum_finish.
+// lane: the wavefronts take rows w, w + W, ..., 32 (4-byte elements) or eight rows' loads in
+// flight per lane (clamped indices; rows past the end add nothing).  Partial sums per wavefront:
+// finish with colsum_finish.
 template <int NT, int CPL, typename T>
 __device__ __forceinline__ void colsum_acc(const T* M, int rows, int cols, const double* v, double (&acc)[CPL]) {
   constexpr int W = NT / 64;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const QPX_GLOBAL_AS T* Mg = as_global(M);
-  // RD rows' loads in flight per lane: sixteen for 4-byte elements, eight for 8-byte
-  constexpr int RD = sizeof(T) <= 4 ? 16 : 8;
+  // RD rows' loads in flight per lane: 32 for 4-byte elements, eight for 8-byte
+  constexpr int RD = sizeof(T) <= 4 ? 32 : 8;
   for (int k0 = w; k0 < rows; k0 += RD * W) {
     T m[RD][CPL];
     double vk[RD];
