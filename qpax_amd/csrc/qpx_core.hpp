@@ -449,11 +449,13 @@ __device__ __forceinline__ void rows_cols16(const MT* M, int rows, int n, const 
     xs[u] = j < n ? x[j] : 0.0;
     ca[u] = 0.0;
   }
-  for (int kb = 4 * w; kb < rows; kb += 8 * W) {
-    MT mv[2][S];
-    double vk[2];
+  // NG groups of four rows per wavefront in flight: four for 4-byte elements, two for 8-byte
+  constexpr int NG = sizeof(MT) <= 4 ? 4 : 2;
+  for (int kb = 4 * w; kb < rows; kb += 4 * NG * W) {
+    MT mv[NG][S];
+    double vk[NG];
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < NG; ++g) {
       const int k0 = kb + 4 * W * g + rq, k = k0 < rows ? k0 : rows - 1;
       const double vv = vg[(unsigned)k];
       vk[g] = k0 < rows ? vv : 0.0;
@@ -464,7 +466,7 @@ __device__ __forceinline__ void rows_cols16(const MT* M, int rows, int n, const 
       }
     }
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < NG; ++g) {
       double a0 = 0.0, a1 = 0.0;
 #pragma unroll
       for (int u = 0; u < S; u += 2) {
